@@ -1,0 +1,187 @@
+/*
+ * vsfeel_bm3d.h -- C ABI of the MI355X (gfx950) BM3Dv2 denoiser.
+ *
+ * Drop-in boundary for the BM3Dv2 filter of TheFeelTrain/vapoursynth-feel.
+ * The reference registers BM3Dv2 in src/vsfeel.cpp:350-364 ->
+ * src/bm3d.cpp:1558-1574 and implements it as BM3DCreate (src/bm3d.cpp:1043),
+ * BM3DGetFrame (:698) and BM3DFree (:1030) over the Vulkan shaders
+ * src/bm3d.comp and src/bm3d_agg.comp.  Each entry point below names the
+ * reference interface it replaces.  Plain C types only: no torch, no HIP
+ * types (streams are passed as void *, a hipStream_t or NULL for the default
+ * stream).  Every call returns VSFEEL_BM3D_OK (0) or an error code, and writes
+ * a message in the reference's "BM3D: ..." style into err (when err_len > 0).
+ *
+ * Thread safety: one handle may be used from several threads; calls on one
+ * handle are serialised internally.  Device-side calls on one handle are
+ * ordered across the streams they are issued on.
+ */
+#ifndef VSFEEL_BM3D_H
+#define VSFEEL_BM3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VSFEEL_BM3D_OK 0
+#define VSFEEL_BM3D_EINVAL 1   /* rejected argument (BM3DCreate's set_error) */
+#define VSFEEL_BM3D_EDEVICE 2  /* HIP runtime error */
+#define VSFEEL_BM3D_ERANGE 3   /* frame index / size outside the clip */
+
+/* VapourSynth-style format of a clip: the fields BM3DCreate inspects
+ * (src/bm3d.cpp:1067-1088, :1257-1275). */
+typedef struct vsfeel_bm3d_format {
+    int32_t color_family;    /* 1 Gray, 2 RGB, 3 YUV (VapourSynth cfGray/cfRGB/cfYUV) */
+    int32_t sample_type;     /* 0 integer, 1 float (stInteger/stFloat) */
+    int32_t bits_per_sample;
+    int32_t sub_sampling_w;
+    int32_t sub_sampling_h;
+    int32_t width;           /* luma plane */
+    int32_t height;
+    int32_t num_frames;
+} vsfeel_bm3d_format;
+
+/*
+ * Arguments of core.vsfeel.BM3Dv2, with the registration string of
+ * src/bm3d.cpp:1561-1571:
+ *   clip:vnode; ref:vnode:opt; sigma:float[]:opt; block_step:int[]:opt;
+ *   bm_range:int[]:opt; radius:int:opt; ps_num:int[]:opt; ps_range:int[]:opt;
+ *   num_streams:int:opt; extractor_exp:int:opt;
+ * Each optional argument is (pointer, count); count 0 means "not given".
+ * Array arguments are read at indices 0..2 like mapGetFloat/mapGetInt
+ * (later planes inherit the previous value); only plane 0's value is used.
+ */
+typedef struct vsfeel_bm3d_args {
+    vsfeel_bm3d_format clip;
+    int32_t has_ref;
+    vsfeel_bm3d_format ref;
+    const double *sigma;          int32_t num_sigma;
+    const int64_t *block_step;    int32_t num_block_step;
+    const int64_t *bm_range;      int32_t num_bm_range;
+    const int64_t *radius;        int32_t num_radius;
+    const int64_t *ps_num;        int32_t num_ps_num;
+    const int64_t *ps_range;      int32_t num_ps_range;
+    const int64_t *num_streams;   int32_t num_num_streams;
+    const int64_t *extractor_exp; int32_t num_extractor_exp;
+    /* Not in the reference's signature: rounding of the shaders' multiply-adds
+     * (0 all separate, 1 GLSL fma() fused [default], 2 every a*b+c fused) and
+     * the HIP device ordinal (the reference always uses device 0). */
+    int32_t fp_mode;
+    int32_t device;
+} vsfeel_bm3d_args;
+
+/* Derived parameters of an instance (what BM3DCreate stores in BM3DData,
+ * src/bm3d.cpp:74-130). */
+typedef struct vsfeel_bm3d_info {
+    int32_t width, height, num_frames;
+    int32_t chroma;           /* YUV input: chroma passes through */
+    int32_t final_pass;       /* a "ref" clip was given (Wiener pass) */
+    float sigma;              /* plane-0 sigma after the reference's scaling */
+    float extractor;          /* 0 or 2^extractor_exp */
+    int32_t block_step, bm_range, radius, ps_num, ps_range, num_streams;
+    int32_t temporal_width;   /* 2 * radius + 1 */
+    int32_t ref_blocks_x, ref_blocks_y;  /* active reference-block grid */
+    int32_t fp_mode;
+    int32_t device;
+    int32_t pitch;            /* device plane pitch (elements) of the internal buffers */
+} vsfeel_bm3d_info;
+
+typedef struct vsfeel_bm3d vsfeel_bm3d;
+
+/*
+ * Replaces BM3DCreate (src/bm3d.cpp:1043-1552): validates the arguments in
+ * the reference's order with its messages ("BM3D: \"radius\" must be in range
+ * [0, 4]", ...), scales sigma, sizes the frame caches and allocates the
+ * device buffers.  *out is NULL on error.
+ */
+int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out,
+                       char *err, size_t err_len);
+
+/* Replaces BM3DFree (src/bm3d.cpp:1030-1037): waits for the device and frees. */
+void vsfeel_bm3d_free(vsfeel_bm3d *h);
+
+int vsfeel_bm3d_get_info(const vsfeel_bm3d *h, vsfeel_bm3d_info *info);
+
+/*
+ * The frames output frame n needs: [clamp(n-2r), clamp(n+2r)], the request
+ * window of BM3DGetFrame's arInitial branch (src/bm3d.cpp:704-712).
+ */
+int vsfeel_bm3d_frame_window(const vsfeel_bm3d *h, int32_t n, int32_t *first, int32_t *last);
+
+/*
+ * Replaces BM3DGetFrame's arAllFramesReady branch (src/bm3d.cpp:713-1025)
+ * for the luma plane.  clip_planes[i] / ref_planes[i] are host pointers to
+ * the luma planes of frames first+i of vsfeel_bm3d_frame_window(n), with row
+ * strides in bytes (ref_* only when the instance has a ref clip, else NULL).
+ * Frames whose data is already cached on the device are not read.  The
+ * denoised luma of frame n is written to dst (host, stride in bytes).
+ * Chroma pass-through and frame allocation stay with the caller (the VS
+ * shim copies planes 1 and 2 like newVideoFrame2, src/bm3d.cpp:715-726).
+ */
+int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n,
+                          const float *const *clip_planes, const ptrdiff_t *clip_strides,
+                          const float *const *ref_planes, const ptrdiff_t *ref_strides,
+                          float *dst, ptrdiff_t dst_stride,
+                          char *err, size_t err_len);
+
+/* A clip resident in device memory: num_frames luma planes. */
+typedef struct vsfeel_bm3d_device_clip {
+    const float *data;        /* frame f starts at data + f * frame_stride */
+    int64_t frame_stride;     /* elements */
+    int32_t pitch;            /* elements between rows (>= width) */
+    const float *ref;         /* ref clip (final pass) with the same layout, or NULL */
+} vsfeel_bm3d_device_clip;
+
+/*
+ * Device-resident counterpart of BM3DGetFrame for a batch of output frames
+ * [first, first + count): the whole per-frame launch sequence (slot clear,
+ * block matching, collaborative filtering, aggregation) is enqueued on
+ * `stream`; estimate stacks are reused across the batch the way the
+ * reference's res cache reuses them across frames (src/bm3d.cpp:380-456).
+ * out + (n - first) * out_frame_stride receives frame n.  Returns after
+ * enqueueing; synchronise the stream before reading out.
+ */
+int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip,
+                               int32_t first, int32_t count,
+                               float *out, int32_t out_pitch, int64_t out_frame_stride,
+                               void *stream, char *err, size_t err_len);
+
+/*
+ * Debug: the groups one estimation dispatch of frame m forms (no
+ * counterpart buffer in the reference, whose shader keeps them in shared
+ * memory, src/bm3d.comp:657-661).  groups receives ref_blocks_y *
+ * ref_blocks_x * 8 (x, y, z) int32 triplets in raster order of the ref
+ * blocks, z being the window position (radius = the frame itself); dists
+ * (optional) the 8 distances (-1 for a block forced in by the self-block rule).
+ * Host buffers; synchronous.
+ */
+int vsfeel_bm3d_debug_groups(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip, int32_t m,
+                             int32_t *groups, float *dists, char *err, size_t err_len);
+
+/*
+ * Debug: the estimate stack of frame m (one dispatch of the estimation
+ * shader, src/bm3d.comp:445-718): (2r+1) x {weighted sum, weight} planes of
+ * height x res_pitch floats, written to the device buffer res.  Synchronous.
+ */
+int vsfeel_bm3d_debug_estimate(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip, int32_t m,
+                               float *res, int32_t res_pitch, char *err, size_t err_len);
+
+/*
+ * Per-kernel GPU timing, the counterpart of the reference's BM3D_GPUTRACE
+ * timestamp queries (src/bm3d.cpp:1492-1501, :978-994): when enabled, HIP
+ * events are recorded around every launch on the launch stream.
+ * vsfeel_bm3d_set_timing resets the counters; vsfeel_bm3d_kernel_times waits
+ * for the recorded launches and returns the summed milliseconds and launch
+ * counts per kernel: [0] slot clear, [1] block matching, [2] collaborative
+ * filtering + scatter, [3] aggregation.
+ */
+int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
+int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSFEEL_BM3D_H */

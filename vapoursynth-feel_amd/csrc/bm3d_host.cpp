@@ -1,0 +1,827 @@
+// Host side of the BM3Dv2 library: argument validation (BM3DCreate), frame
+// caches and per-frame launch sequences (BM3DGetFrame), behind the C ABI of
+// include/vsfeel_bm3d.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bm3d_common.hpp"
+#include "vsfeel_bm3d.h"
+
+namespace vsfeel {
+
+hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s);
+hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, hipStream_t s);
+hipError_t launch_aggregate(const AggArgs &a, hipStream_t s);
+hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
+
+namespace {
+
+// sigma factors of BM3DCreate (src/bm3d.cpp:1105-1107): 0x3f021bb6 for the
+// basic pass, 0x3e40c0c1 for the final (Wiener) pass
+float bits_to_float(uint32_t u)
+{
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// vsh::int64ToIntS: saturate to int
+int sat_int(int64_t v)
+{
+    if (v > INT_MAX) {
+        return INT_MAX;
+    }
+    if (v < INT_MIN) {
+        return INT_MIN;
+    }
+    return int(v);
+}
+
+int clampi(int v, int lo, int hi) { return std::min(std::max(v, lo), hi); }
+
+// slice of frame clamp(n-r+i)'s stack that lands on frame n (src/bm3d.cpp:369-371)
+int agg_z(int i, int n, int nframes, int radius)
+{
+    return std::min(std::max(2 * radius - i, n - nframes + 1 + radius), n + radius);
+}
+
+enum KernelKind { kKernelFill = 0, kKernelMatch, kKernelEstimate, kKernelAggregate, kKernelKinds };
+
+}  // namespace
+
+struct Instance {
+    // parameters (BM3DData, src/bm3d.cpp:74-130)
+    int width = 0, height = 0, nframes = 0;
+    bool chroma = false, final_pass = false, process = true;
+    float sigma = 0, sigma_u = 0, sigma_v = 0, extractor = 0;
+    int block_step = 8, bm_range = 9, radius = 0, ps_num = 2, ps_range = 4, num_streams = 4;
+    int fp_mode = kFpExplicit, device = 0;
+    int tw = 1, src_ring = 1, res_cap = 1, clips = 1;
+    int nx = 0, ny = 0, nref = 0;
+    int pitch = 0;     // elements, plane pitch of the internal buffers
+    long long pe = 0;  // height * pitch
+
+    float *src_buf = nullptr;    // src_ring * clips planes (ref clip first in final mode)
+    float *res_buf = nullptr;    // res_cap stacks of tw * 2 planes
+    uint32_t *groups = nullptr;  // nref * 8
+    float *dists = nullptr;      // nref * 8 (debug)
+    float *dst_dev = nullptr;    // one plane
+    hipStream_t stream = nullptr;
+    hipEvent_t last_done = nullptr;  // end of the previous device-side call
+    bool have_last = false;
+
+    std::vector<int> src_frame;  // frame held by each src ring slot (-1: none)
+    std::vector<int> res_frame;  // frame whose estimate stack each res slot holds
+    std::mutex mu;
+
+    // optional per-kernel timing: HIP events recorded around every launch on
+    // the launch stream, harvested on request
+    struct Timed {
+        hipEvent_t a, b;
+        int kind;
+    };
+    bool timing = false;
+    std::vector<Timed> timed;
+    std::vector<hipEvent_t> event_pool;
+    double kernel_ms[kKernelKinds] = {};
+    long long kernel_count[kKernelKinds] = {};
+
+    hipEvent_t take_event()
+    {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) {
+            return nullptr;
+        }
+        return e;
+    }
+
+    hipError_t harvest()
+    {
+        hipError_t rc = hipSuccess;
+        for (const Timed &t : timed) {
+            float ms = 0.0f;
+            hipError_t e = hipEventSynchronize(t.b);
+            if (e == hipSuccess) {
+                e = hipEventElapsedTime(&ms, t.a, t.b);
+            }
+            if (e == hipSuccess) {
+                kernel_ms[t.kind] += ms;
+                kernel_count[t.kind] += 1;
+            } else {
+                rc = e;
+            }
+            event_pool.push_back(t.a);
+            event_pool.push_back(t.b);
+        }
+        timed.clear();
+        return rc;
+    }
+
+    template <class F>
+    hipError_t timed_launch(int kind, hipStream_t s, F &&launch)
+    {
+        if (!timing) {
+            return launch();
+        }
+        if (timed.size() >= 4096) {
+            hipError_t e = harvest();
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
+        hipEvent_t a = take_event(), b = take_event();
+        if (!a || !b) {
+            return hipErrorOutOfMemory;
+        }
+        hipError_t e = hipEventRecord(a, s);
+        if (e == hipSuccess) {
+            e = launch();
+        }
+        if (e == hipSuccess) {
+            e = hipEventRecord(b, s);
+        }
+        timed.push_back({ a, b, kind });
+        return e;
+    }
+
+    ~Instance()
+    {
+        if (device < 0) {
+            return;  // creation failed before any device object existed
+        }
+        (void)hipSetDevice(device);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+        }
+        (void)hipFree(src_buf);
+        (void)hipFree(res_buf);
+        (void)hipFree(groups);
+        (void)hipFree(dists);
+        (void)hipFree(dst_dev);
+        if (last_done) {
+            (void)hipEventDestroy(last_done);
+        }
+        (void)harvest();
+        for (hipEvent_t e : event_pool) {
+            (void)hipEventDestroy(e);
+        }
+        if (stream) {
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    float *src_slot(int slot, int clip) const { return src_buf + ((long long)slot * clips + clip) * pe; }
+    float *res_slot(int slot) const { return res_buf + (long long)slot * tw * 2 * pe; }
+
+    // ------------------------------------------------------------------
+    // one estimation dispatch (record_bm3d_kernels, src/bm3d.cpp:558-602):
+    // clear the stack, block matching, collaborative filtering
+    // ------------------------------------------------------------------
+    hipError_t estimate(const float *const *search, const float *const *input, int in_pitch,
+                        float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr)
+    {
+        hipError_t e = timed_launch(kKernelFill, s, [&] {
+            return launch_fill_zero(stack, (long long)tw * 2 * height * stack_pitch, s);
+        });
+        if (e != hipSuccess) {
+            return e;
+        }
+        MatchArgs m {};
+        for (int z = 0; z < tw; ++z) {
+            m.plane[z] = search[z];
+        }
+        m.groups = groups;
+        m.dists = dists_out;
+        m.width = width;
+        m.height = height;
+        m.pitch = in_pitch;
+        m.step = block_step;
+        m.bm_range = bm_range;
+        m.radius = radius;
+        m.ps_num = ps_num;
+        m.ps_range = ps_range;
+        m.nx = nx;
+        m.ny = ny;
+        e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
+        if (e != hipSuccess) {
+            return e;
+        }
+        EstimateArgs a {};
+        for (int z = 0; z < tw; ++z) {
+            a.search[z] = search[z];
+            a.input[z] = input[z];
+        }
+        a.groups = groups;
+        a.res = stack;
+        a.height = height;
+        a.pitch = in_pitch;
+        a.res_pitch = stack_pitch;
+        a.ngroups = nref;
+        a.sigma = sigma;
+        a.extractor = extractor;
+        return timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, s); });
+    }
+
+    // aggregation of frame n from the res slots (record_bm3d_agg, src/bm3d.cpp:628-696)
+    hipError_t aggregate(int n, const std::vector<int> &slot_of, float *dst, int dst_pitch, hipStream_t s)
+    {
+        AggArgs a {};
+        for (int i = 0; i < tw; ++i) {
+            const int z = radius == 0 ? 0 : agg_z(i, n, nframes, radius);
+            a.slice[i] = res_slot(slot_of[i]) + (long long)z * 2 * pe;
+        }
+        a.dst = dst;
+        a.plane_elems = pe;
+        a.width = width;
+        a.height = height;
+        a.pitch = pitch;
+        a.dst_pitch = dst_pitch;
+        a.tw = tw;
+        return timed_launch(kKernelAggregate, s, [&] { return launch_aggregate(a, s); });
+    }
+
+    // order this call's device work after the previous call's
+    hipError_t begin_on(hipStream_t s)
+    {
+        if (have_last) {
+            return hipStreamWaitEvent(s, last_done, 0);
+        }
+        return hipSuccess;
+    }
+
+    hipError_t end_on(hipStream_t s)
+    {
+        hipError_t e = hipEventRecord(last_done, s);
+        have_last = e == hipSuccess;
+        return e;
+    }
+};
+
+namespace {
+
+void set_err(char *err, size_t len, const std::string &msg)
+{
+    if (err && len > 0) {
+        std::snprintf(err, len, "%s", msg.c_str());
+    }
+}
+
+int fail(char *err, size_t len, int code, const std::string &msg)
+{
+    set_err(err, len, "BM3D: " + msg);
+    return code;
+}
+
+int hip_fail(char *err, size_t len, hipError_t e, const char *what)
+{
+    return fail(err, len, VSFEEL_BM3D_EDEVICE, std::string(what) + " failed: " + hipGetErrorString(e));
+}
+
+#define VSFEEL_HIP(call, what)                                   \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) {                                  \
+            return hip_fail(err, err_len, e_, what);             \
+        }                                                        \
+    } while (0)
+
+// BM3DCreate's parameter parsing (src/bm3d.cpp:1062-1192) and plane setup
+// (:1256-1275); returns an empty string or the message without the prefix
+std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
+{
+    const vsfeel_bm3d_format &f = a.clip;
+    if (a.has_ref) {
+        const vsfeel_bm3d_format &r = a.ref;
+        if (r.color_family != f.color_family || r.sample_type != f.sample_type ||
+            r.bits_per_sample != f.bits_per_sample || r.sub_sampling_w != f.sub_sampling_w ||
+            r.sub_sampling_h != f.sub_sampling_h) {
+            return "\"ref\" must be of the same format as \"clip\"";
+        }
+        if (r.width != f.width || r.height != f.height) {
+            return "\"ref\" must be of the same dimensions as \"clip\"";
+        }
+        if (r.num_frames != f.num_frames) {
+            return "\"ref\" must be of the same number of frames as \"clip\"";
+        }
+        d.final_pass = true;
+    }
+    if (f.width <= 0 || f.height <= 0 || f.sample_type != 1 || f.bits_per_sample != 32) {
+        return "only constant format 32 bit float input supported";
+    }
+
+    float sigma[3];
+    for (int i = 0; i < 3; ++i) {
+        if (i < a.num_sigma && a.sigma) {
+            sigma[i] = float(a.sigma[i]);
+            if (sigma[i] < 0.0f) {
+                return "\"sigma\" must be non-negative";
+            }
+        } else {
+            sigma[i] = i == 0 ? 3.0f : sigma[i - 1];
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        d.process = sigma[i] >= FLT_EPSILON;
+    }
+    const float factor = d.final_pass ? bits_to_float(0x3e40c0c1u) : bits_to_float(0x3f021bb6u);
+    for (float &s : sigma) {
+        s *= factor;
+    }
+    d.sigma = sigma[0];
+    d.sigma_u = sigma[1];
+    d.sigma_v = sigma[2];
+
+    auto int_array = [](const int64_t *v, int n, int def, int *out, auto bad, const char *msg) -> std::string {
+        for (int i = 0; i < 3; ++i) {
+            if (i < n && v) {
+                out[i] = sat_int(v[i]);
+                if (bad(out[i])) {
+                    return msg;
+                }
+            } else {
+                out[i] = i == 0 ? def : out[i - 1];
+            }
+        }
+        return "";
+    };
+    int tmp[3];
+    std::string e;
+    e = int_array(a.block_step, a.num_block_step, 8, tmp, [](int v) { return v <= 0 || v > 8; },
+                  "\"block_step\" must be in range [1, 8]");
+    if (!e.empty()) {
+        return e;
+    }
+    d.block_step = tmp[0];
+    e = int_array(a.bm_range, a.num_bm_range, 9, tmp, [](int v) { return v <= 0; },
+                  "\"bm_range\" must be positive");
+    if (!e.empty()) {
+        return e;
+    }
+    d.bm_range = tmp[0];
+    d.radius = (a.num_radius > 0 && a.radius) ? sat_int(a.radius[0]) : 0;
+    if (d.radius < 0 || d.radius > 4) {
+        return "\"radius\" must be in range [0, 4]";
+    }
+    d.tw = 2 * d.radius + 1;
+    e = int_array(a.ps_num, a.num_ps_num, 2, tmp, [](int v) { return v <= 0 || v > 8; },
+                  "\"ps_num\" must be in range [1, 8]");
+    if (!e.empty()) {
+        return e;
+    }
+    d.ps_num = tmp[0];
+    e = int_array(a.ps_range, a.num_ps_range, 4, tmp, [](int v) { return v <= 0; },
+                  "\"ps_range\" must be positive");
+    if (!e.empty()) {
+        return e;
+    }
+    d.ps_range = tmp[0];
+    d.num_streams = (a.num_num_streams > 0 && a.num_streams) ? sat_int(a.num_streams[0]) : 4;
+    if (d.num_streams <= 0) {
+        return "\"num_streams\" must be positive";
+    }
+    // cache capacities of src/bm3d.cpp:1185-1186
+    d.src_ring = d.radius == 0 ? d.num_streams : 4 * d.radius + d.num_streams;
+    d.res_cap = d.radius == 0 ? d.num_streams : d.tw + d.num_streams + 2 * d.radius;
+    const int ex = (a.num_extractor_exp > 0 && a.extractor_exp) ? sat_int(a.extractor_exp[0]) : 0;
+    d.extractor = ex != 0 ? std::ldexp(1.0f, ex) : 0.0f;
+    d.nframes = f.num_frames;
+
+    if (f.color_family == 1) {
+        d.chroma = false;
+    } else if (f.color_family == 3) {
+        d.chroma = true;
+    } else {
+        // the reference's message carries the prefix twice (src/bm3d.cpp:1274)
+        return "BM3D: only Gray and YUV input are currently supported";
+    }
+    // not checked by the reference, whose shader then reads outside the plane
+    if (f.width < 8 || f.height < 8) {
+        return "the luma plane must be at least 8x8";
+    }
+    if (f.width > kMaxDim || f.height > kMaxDim) {
+        return "the luma plane must be at most 16383x16383";
+    }
+    if (f.num_frames <= 0) {
+        return "the clip must have a known, positive number of frames";
+    }
+    d.fp_mode = a.fp_mode;
+    if (d.fp_mode < 0 || d.fp_mode > 2) {
+        return "\"fp_mode\" must be 0, 1 or 2";
+    }
+    d.device = a.device;
+    d.width = f.width;
+    d.height = f.height;
+    d.clips = d.final_pass ? 2 : 1;
+    d.nx = (d.width - 8 + d.block_step + d.block_step - 1) / d.block_step;
+    d.ny = (d.height - 8 + d.block_step + d.block_step - 1) / d.block_step;
+    d.nref = d.nx * d.ny;
+    d.pitch = (d.width + 63) & ~63;
+    d.pe = (long long)d.height * d.pitch;
+    return "";
+}
+
+}  // namespace
+}  // namespace vsfeel
+
+struct vsfeel_bm3d {
+    vsfeel::Instance inst;
+};
+
+using vsfeel::Instance;
+
+extern "C" {
+
+int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!out || !args) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null argument");
+    }
+    *out = nullptr;
+    auto *h = new vsfeel_bm3d();
+    Instance &d = h->inst;
+    d.device = -1;
+    const std::string msg = parse_args(*args, d);
+    if (!msg.empty()) {
+        delete h;
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, msg);
+    }
+    auto dev_fail = [&](hipError_t e, const char *what) {
+        const int rc = hip_fail(err, err_len, e, what);
+        delete h;
+        return rc;
+    };
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= d.device || d.device < 0) {
+        const int rc = fail(err, err_len, VSFEEL_BM3D_EDEVICE,
+                            e != hipSuccess ? std::string("no HIP device: ") + hipGetErrorString(e)
+                                            : std::string("HIP device ") + std::to_string(d.device) +
+                                                  " is not available");
+        d.device = -1;
+        delete h;
+        return rc;
+    }
+    if ((e = hipSetDevice(d.device)) != hipSuccess) {
+        return dev_fail(e, "hipSetDevice");
+    }
+    if ((e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking)) != hipSuccess) {
+        return dev_fail(e, "hipStreamCreate");
+    }
+    if ((e = hipEventCreateWithFlags(&d.last_done, hipEventDisableTiming)) != hipSuccess) {
+        return dev_fail(e, "hipEventCreate");
+    }
+    const size_t plane = size_t(d.pe) * sizeof(float);
+    if ((e = hipMalloc(&d.src_buf, plane * d.src_ring * d.clips)) != hipSuccess ||
+        (e = hipMalloc(&d.res_buf, plane * d.res_cap * d.tw * 2)) != hipSuccess ||
+        (e = hipMalloc(&d.groups, size_t(d.nref) * 8 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc(&d.dists, size_t(d.nref) * 8 * sizeof(float))) != hipSuccess ||
+        (e = hipMalloc(&d.dst_dev, plane)) != hipSuccess) {
+        return dev_fail(e, "hipMalloc");
+    }
+    d.src_frame.assign(d.src_ring, -1);
+    d.res_frame.assign(d.res_cap, -1);
+    *out = h;
+    return VSFEEL_BM3D_OK;
+}
+
+void vsfeel_bm3d_free(vsfeel_bm3d *h) { delete h; }
+
+int vsfeel_bm3d_get_info(const vsfeel_bm3d *h, vsfeel_bm3d_info *info)
+{
+    if (!h || !info) {
+        return VSFEEL_BM3D_EINVAL;
+    }
+    const Instance &d = h->inst;
+    info->width = d.width;
+    info->height = d.height;
+    info->num_frames = d.nframes;
+    info->chroma = d.chroma;
+    info->final_pass = d.final_pass;
+    info->sigma = d.sigma;
+    info->extractor = d.extractor;
+    info->block_step = d.block_step;
+    info->bm_range = d.bm_range;
+    info->radius = d.radius;
+    info->ps_num = d.ps_num;
+    info->ps_range = d.ps_range;
+    info->num_streams = d.num_streams;
+    info->temporal_width = d.tw;
+    info->ref_blocks_x = d.nx;
+    info->ref_blocks_y = d.ny;
+    info->fp_mode = d.fp_mode;
+    info->device = d.device;
+    info->pitch = d.pitch;
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_frame_window(const vsfeel_bm3d *h, int32_t n, int32_t *first, int32_t *last)
+{
+    if (!h || !first || !last) {
+        return VSFEEL_BM3D_EINVAL;
+    }
+    const Instance &d = h->inst;
+    if (n < 0 || n >= d.nframes) {
+        return VSFEEL_BM3D_ERANGE;
+    }
+    *first = vsfeel::clampi(n - 2 * d.radius, 0, d.nframes - 1);
+    *last = vsfeel::clampi(n + 2 * d.radius, 0, d.nframes - 1);
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_planes,
+                          const ptrdiff_t *clip_strides, const float *const *ref_planes,
+                          const ptrdiff_t *ref_strides, float *dst, ptrdiff_t dst_stride, char *err,
+                          size_t err_len)
+{
+    using namespace vsfeel;
+    if (!h) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null handle");
+    }
+    Instance &d = h->inst;
+    if (n < 0 || n >= d.nframes) {
+        return fail(err, err_len, VSFEEL_BM3D_ERANGE, "frame " + std::to_string(n) + " is out of range");
+    }
+    if (!clip_planes || !clip_strides || !dst || (d.final_pass && (!ref_planes || !ref_strides))) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing frame planes");
+    }
+    std::lock_guard<std::mutex> lock(d.mu);
+    VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t s = d.stream;
+    VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
+    const int r = d.radius, nf = d.nframes;
+    const int lo = clampi(n - 2 * r, 0, nf - 1), hi = clampi(n + 2 * r, 0, nf - 1);
+    const size_t row_bytes = size_t(d.width) * sizeof(float);
+
+    // upload the window frames the ring does not hold yet (src/bm3d.cpp:772-814);
+    // at radius 0 the single slot is rewritten for every frame like the
+    // reference's per-stream slot
+    for (int f = lo; f <= hi; ++f) {
+        const int slot = r == 0 ? 0 : f % d.src_ring;
+        if (r > 0 && d.src_frame[slot] == f) {
+            continue;
+        }
+        const int i = f - lo;
+        if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
+            return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing plane of frame " + std::to_string(f));
+        }
+        // in the final pass the slot holds [ref][source] like the reference's ring
+        VSFEEL_HIP(hipMemcpy2DAsync(d.src_slot(slot, d.clips - 1), size_t(d.pitch) * sizeof(float), clip_planes[i],
+                                    size_t(clip_strides[i]), row_bytes, size_t(d.height), hipMemcpyHostToDevice, s),
+                   "hipMemcpy2DAsync");
+        if (d.final_pass) {
+            VSFEEL_HIP(hipMemcpy2DAsync(d.src_slot(slot, 0), size_t(d.pitch) * sizeof(float), ref_planes[i],
+                                        size_t(ref_strides[i]), row_bytes, size_t(d.height), hipMemcpyHostToDevice,
+                                        s),
+                       "hipMemcpy2DAsync");
+        }
+        d.src_frame[slot] = f;
+    }
+
+    // estimate stacks of the window frames (src/bm3d.cpp:558-602)
+    std::vector<int> slot_of(d.tw);
+    for (int i = 0; i < d.tw; ++i) {
+        const int m = clampi(n - r + i, 0, nf - 1);
+        const int slot = r == 0 ? 0 : m % d.res_cap;
+        slot_of[i] = slot;
+        if (r > 0 && d.res_frame[slot] == m) {
+            continue;
+        }
+        const float *search[kMaxWindow];
+        const float *input[kMaxWindow];
+        for (int z = 0; z < d.tw; ++z) {
+            const int fz = clampi(m - r + z, 0, nf - 1);
+            const int ring = r == 0 ? 0 : fz % d.src_ring;
+            search[z] = d.src_slot(ring, 0);
+            input[z] = d.src_slot(ring, d.clips - 1);
+        }
+        VSFEEL_HIP(d.estimate(search, input, d.pitch, d.res_slot(slot), d.pitch, s), "estimate launch");
+        d.res_frame[slot] = r == 0 ? -1 : m;
+    }
+    VSFEEL_HIP(d.aggregate(n, slot_of, d.dst_dev, d.pitch, s), "aggregate launch");
+    VSFEEL_HIP(hipMemcpy2DAsync(dst, size_t(dst_stride), d.dst_dev, size_t(d.pitch) * sizeof(float), row_bytes,
+                                size_t(d.height), hipMemcpyDeviceToHost, s),
+               "hipMemcpy2DAsync");
+    VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    VSFEEL_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return VSFEEL_BM3D_OK;
+}
+
+static int check_device_clip(const Instance &d, const vsfeel_bm3d_device_clip *clip, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!clip || !clip->data) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing device clip");
+    }
+    if (clip->pitch < d.width || clip->frame_stride < (int64_t)clip->pitch * (d.height - 1) + d.width) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "device clip pitch or frame stride too small");
+    }
+    if (d.final_pass && !clip->ref) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "the final pass needs the ref clip");
+    }
+    return VSFEEL_BM3D_OK;
+}
+
+static void device_window(const Instance &d, const vsfeel_bm3d_device_clip *clip, int m, const float **search,
+                          const float **input)
+{
+    for (int z = 0; z < d.tw; ++z) {
+        const int f = vsfeel::clampi(m - d.radius + z, 0, d.nframes - 1);
+        const float *c = clip->data + (int64_t)f * clip->frame_stride;
+        input[z] = c;
+        search[z] = d.final_pass ? clip->ref + (int64_t)f * clip->frame_stride : c;
+    }
+}
+
+int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip, int32_t first, int32_t count,
+                               float *out, int32_t out_pitch, int64_t out_frame_stride, void *stream, char *err,
+                               size_t err_len)
+{
+    using namespace vsfeel;
+    if (!h) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null handle");
+    }
+    Instance &d = h->inst;
+    int rc = check_device_clip(d, clip, err, err_len);
+    if (rc != VSFEEL_BM3D_OK) {
+        return rc;
+    }
+    if (first < 0 || count < 0 || first + count > d.nframes) {
+        return fail(err, err_len, VSFEEL_BM3D_ERANGE, "frame range outside the clip");
+    }
+    if (!out || out_pitch < d.width) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "bad output buffer");
+    }
+    std::lock_guard<std::mutex> lock(d.mu);
+    VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
+    const int r = d.radius, nf = d.nframes;
+    // estimate stacks are only reused inside one call: the caller owns the
+    // clip memory and may rewrite it between calls
+    std::vector<int> held(d.res_cap, -1);
+    std::vector<int> slot_of(d.tw);
+    for (int n = first; n < first + count; ++n) {
+        for (int i = 0; i < d.tw; ++i) {
+            const int m = clampi(n - r + i, 0, nf - 1);
+            const int slot = m % d.res_cap;
+            slot_of[i] = slot;
+            if (held[slot] == m) {
+                continue;
+            }
+            const float *search[kMaxWindow];
+            const float *input[kMaxWindow];
+            device_window(d, clip, m, search, input);
+            VSFEEL_HIP(d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, s), "estimate launch");
+            held[slot] = m;
+        }
+        VSFEEL_HIP(d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, s),
+                   "aggregate launch");
+    }
+    // the res slots now hold this call's stacks, not the host path's frames
+    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_debug_groups(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip, int32_t m, int32_t *groups,
+                             float *dists, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!h) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null handle");
+    }
+    Instance &d = h->inst;
+    int rc = check_device_clip(d, clip, err, err_len);
+    if (rc != VSFEEL_BM3D_OK) {
+        return rc;
+    }
+    if (m < 0 || m >= d.nframes) {
+        return fail(err, err_len, VSFEEL_BM3D_ERANGE, "frame out of range");
+    }
+    std::lock_guard<std::mutex> lock(d.mu);
+    VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t s = d.stream;
+    VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
+    const float *search[kMaxWindow];
+    const float *input[kMaxWindow];
+    device_window(d, clip, m, search, input);
+    MatchArgs a {};
+    for (int z = 0; z < d.tw; ++z) {
+        a.plane[z] = search[z];
+    }
+    a.groups = d.groups;
+    a.dists = d.dists;
+    a.width = d.width;
+    a.height = d.height;
+    a.pitch = clip->pitch;
+    a.step = d.block_step;
+    a.bm_range = d.bm_range;
+    a.radius = d.radius;
+    a.ps_num = d.ps_num;
+    a.ps_range = d.ps_range;
+    a.nx = d.nx;
+    a.ny = d.ny;
+    VSFEEL_HIP(launch_match(a, d.fp_mode, s), "match launch");
+    std::vector<uint32_t> packed(size_t(d.nref) * 8);
+    VSFEEL_HIP(hipMemcpyAsync(packed.data(), d.groups, packed.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpy");
+    if (dists) {
+        VSFEEL_HIP(hipMemcpyAsync(dists, d.dists, packed.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpy");
+    }
+    VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    VSFEEL_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (groups) {
+        for (size_t i = 0; i < packed.size(); ++i) {
+            groups[i * 3 + 0] = int32_t(entry_x(packed[i]));
+            groups[i * 3 + 1] = int32_t(entry_y(packed[i]));
+            groups[i * 3 + 2] = int32_t(entry_z(packed[i]));
+        }
+    }
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_debug_estimate(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip, int32_t m, float *res,
+                               int32_t res_pitch, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!h) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null handle");
+    }
+    Instance &d = h->inst;
+    int rc = check_device_clip(d, clip, err, err_len);
+    if (rc != VSFEEL_BM3D_OK) {
+        return rc;
+    }
+    if (m < 0 || m >= d.nframes) {
+        return fail(err, err_len, VSFEEL_BM3D_ERANGE, "frame out of range");
+    }
+    if (!res || res_pitch < d.width || (res_pitch % 4) != 0) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "bad estimate buffer (pitch must be a multiple of 4)");
+    }
+    std::lock_guard<std::mutex> lock(d.mu);
+    VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t s = d.stream;
+    VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
+    const float *search[kMaxWindow];
+    const float *input[kMaxWindow];
+    device_window(d, clip, m, search, input);
+    VSFEEL_HIP(d.estimate(search, input, clip->pitch, res, res_pitch, s), "estimate launch");
+    VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    VSFEEL_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable)
+{
+    if (!h) {
+        return VSFEEL_BM3D_EINVAL;
+    }
+    Instance &d = h->inst;
+    std::lock_guard<std::mutex> lock(d.mu);
+    if (d.device >= 0) {
+        (void)hipSetDevice(d.device);
+    }
+    (void)d.harvest();
+    for (int k = 0; k < vsfeel::kKernelKinds; ++k) {
+        d.kernel_ms[k] = 0.0;
+        d.kernel_count[k] = 0;
+    }
+    d.timing = enable != 0;
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count)
+{
+    if (!h || !ms || !count) {
+        return VSFEEL_BM3D_EINVAL;
+    }
+    Instance &d = h->inst;
+    std::lock_guard<std::mutex> lock(d.mu);
+    if (d.device >= 0) {
+        (void)hipSetDevice(d.device);
+    }
+    const hipError_t e = d.harvest();
+    for (int k = 0; k < vsfeel::kKernelKinds; ++k) {
+        ms[k] = d.kernel_ms[k];
+        count[k] = d.kernel_count[k];
+    }
+    return e == hipSuccess ? VSFEEL_BM3D_OK : VSFEEL_BM3D_EDEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,796 @@
+// BM3Dv2 kernels for gfx950 (CDNA4, wave64).
+//
+//   bm3d_match     block matching: spatial SSD search plus the temporal
+//                  predictive search, one wavefront per reference block
+//   bm3d_estimate  3-D DCT, hard threshold (or Wiener shrinkage), inverse,
+//                  weighted scatter into the estimate stack
+//   bm3d_aggregate per-pixel sum of the 2r+1 slices and the fp64 division
+//   fill_zero      clears estimate slots (a kernel, so it can sit in a graph)
+//
+// The whole file is compiled with -ffp-contract=off: every multiply-add that
+// must be fused is written as __builtin_fmaf, everything else rounds twice,
+// which is what keeps distances, coefficients and thresholds bit-identical to
+// the CPU restatement of the reference shaders.
+
+#include <float.h>
+
+#include "bm3d_common.hpp"
+
+namespace vsfeel {
+
+namespace {
+
+constexpr uint64_t kPlaceholderKey = (uint64_t(0x7F7FFFFFu) << 32) | 0x7FFFFFFFu;  // (FLT_MAX, INT_MAX)
+constexpr uint64_t kNoKey = ~uint64_t(0);
+constexpr int kSortThreshold = 8;  // survivors above this are merged by a sort
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Orders LDS traffic of one wavefront: LDS operations of a wave execute in
+// issue order, so only the compiler has to be kept from moving them across.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t make_key(float d, int seq)
+{
+    return (uint64_t(__float_as_uint(d)) << 32) | uint32_t(seq);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m)
+{
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(v)), m));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(v >> 32)), m));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
+{
+    const uint32_t lo = uint32_t(__shfl(int(uint32_t(v)), src));
+    const uint32_t hi = uint32_t(__shfl(int(uint32_t(v >> 32)), src));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d)
+{
+    const uint32_t lo = uint32_t(__shfl_up(int(uint32_t(v)), d));
+    const uint32_t hi = uint32_t(__shfl_up(int(uint32_t(v >> 32)), d));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l)
+{
+    return uint32_t(__builtin_amdgcn_readlane(int(v), l));
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l)
+{
+    return (uint64_t(readlane_u32(uint32_t(v >> 32), l)) << 32) | readlane_u32(uint32_t(v), l);
+}
+
+// ---------------------------------------------------------------------------
+// Top-8 list held by lanes 0..7 of a wavefront, ascending by the key
+// (distance bits, seq).  Distances are non-negative, so their IEEE bits order
+// like the values; seq is unique per candidate of one search, so the key
+// order is exactly the reference's (distance, then smaller seq) merge order.
+// ---------------------------------------------------------------------------
+struct TopList {
+    uint64_t key;
+    uint32_t xy;  // x | y << 16
+};
+
+__device__ __forceinline__ void list_reset(TopList &L)
+{
+    L.key = kPlaceholderKey;
+    L.xy = 0;
+}
+
+__device__ __forceinline__ uint64_t list_threshold(const TopList &L) { return readlane_u64(L.key, 7); }
+
+// insert one wave-uniform candidate that is below the current threshold
+__device__ __forceinline__ void list_insert(TopList &L, uint64_t key, uint32_t xy, int lane)
+{
+    const uint64_t below = __ballot(lane < 8 && L.key < key);
+    const int pos = __popcll(below & 0xFFull);
+    const uint64_t up_key = shfl_up64(L.key, 1);
+    const uint32_t up_xy = uint32_t(__shfl_up(int(L.xy), 1));
+    if (lane == pos) {
+        L.key = key;
+        L.xy = xy;
+    } else if (lane > pos && lane < 8) {
+        L.key = up_key;
+        L.xy = up_xy;
+    }
+}
+
+// merge one candidate per lane (kNoKey where there is none) into the list:
+// bitonic sort of the 64 lanes, then the 8 lowest of list and batch
+__device__ void list_merge_batch(TopList &L, uint64_t key, uint32_t xy, int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t ok = shfl_xor64(key, j);
+            const uint32_t oxy = uint32_t(__shfl_xor(int(xy), j));
+            const bool asc = (lane & k) == 0;
+            const bool lower = (lane & j) == 0;
+            const bool take = (lower == asc) ? (ok < key) : (ok > key);
+            if (take) {
+                key = ok;
+                xy = oxy;
+            }
+        }
+    }
+    const int rev = (7 - lane) & 63;
+    const uint64_t bk = shfl64(key, rev);
+    const uint32_t bxy = uint32_t(__shfl(int(xy), rev));
+    uint64_t ck = L.key;
+    uint32_t cxy = L.xy;
+    if (bk < ck) {
+        ck = bk;
+        cxy = bxy;
+    }
+#pragma unroll
+    for (int j = 4; j > 0; j >>= 1) {
+        const uint64_t ok = shfl_xor64(ck, j);
+        const uint32_t oxy = uint32_t(__shfl_xor(int(cxy), j));
+        const bool lower = (lane & j) == 0;
+        const bool take = lower ? (ok < ck) : (ok > ck);
+        if (take) {
+            ck = ok;
+            cxy = oxy;
+        }
+    }
+    if (lane < 8) {
+        L.key = ck;
+        L.xy = cxy;
+    }
+}
+
+// offer one candidate per lane (valid where `cand`) to the list
+__device__ __forceinline__ void list_offer(TopList &L, bool cand, float d, int seq, uint32_t xy, int lane)
+{
+    const bool valid = cand && (d < FLT_MAX);  // the shader's strict '<' against FLT_MAX
+    const uint64_t key = valid ? make_key(d, seq) : kNoKey;
+    const uint64_t thr = list_threshold(L);
+    const bool surv = key < thr;
+    uint64_t mask = __ballot(surv);
+    if (mask == 0) {
+        return;
+    }
+    if (__popcll(mask) > kSortThreshold) {
+        list_merge_batch(L, surv ? key : kNoKey, xy, lane);
+        return;
+    }
+    while (mask) {
+        const int src = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        const uint64_t k = readlane_u64(key, src);
+        const uint32_t kxy = readlane_u32(xy, src);
+        if (k < list_threshold(L)) {
+            list_insert(L, k, kxy, lane);
+        }
+    }
+}
+
+// squared error of one 8-row column, accumulated like col_ssd of the shader
+// (even and odd rows in two partial sums)
+template <int CM>
+__device__ __forceinline__ float col_ssd(const float *c, const float *s)
+{
+    float e0 = 0.0f, e1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+        const float v0 = c[i] - s[i];
+        const float v1 = c[i + 1] - s[i + 1];
+        if (CM == kFpAll) {
+            e0 = __builtin_fmaf(v0, v0, e0);
+            e1 = __builtin_fmaf(v1, v1, e1);
+        } else {
+            e0 = e0 + v0 * v0;
+            e1 = e1 + v1 * v1;
+        }
+    }
+    return e0 + e1;
+}
+
+// ---------------------------------------------------------------------------
+// One search window.  Lanes map to window columns: lane j of a row segment
+// computes one column SSD, with the reference-block column min(j, 7) for the
+// sliding metric (radius < 3), and a row's candidate k is the tree sum of the
+// column values of lanes k..k+7 -- exactly what the shader's shifted column
+// sums add up to.  Short rows share the wave (several row segments per pass);
+// long rows are cut into chunks of 57 candidates.  Each lane walks down its
+// column with an 8-row register window.
+// ---------------------------------------------------------------------------
+template <int CM, bool EXACT>
+__device__ void scan_window(TopList &L, const float *__restrict__ plane, int pitch, int height,
+                            const float *cur_lds, const float *cur_reg, int left, int right,
+                            int top, int bottom, int seq_base, int lane)
+{
+    const int rw = right - left + 1;
+    const int rh = bottom - top + 1;
+    for (int k0 = 0; k0 < rw; k0 += 57) {
+        const int nc = min(rw - k0, 57);
+        const int seg_len = nc + 7;
+        const int per_pass = 64 / seg_len;
+        const int seg = lane / seg_len;
+        const int kk = lane - seg * seg_len;
+        const bool lane_on = seg < per_pass;
+        const int j = k0 + kk;
+        const int X = lane_on ? left + j : left;
+        float c[8];
+        if (!EXACT) {
+            const int a = min(j, 7);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                c[i] = cur_lds[a * 8 + i];
+            }
+        }
+        const float *col_ptr = plane + X;
+        int cy = top + (lane_on ? seg : 0);
+        float s[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            s[i] = col_ptr[(long long)min(cy + i, height - 1) * pitch];
+        }
+        const int npass = (rh + per_pass - 1) / per_pass;
+        for (int pass = 0; pass < npass; ++pass) {
+            if (pass > 0) {
+                for (int q = 0; q < per_pass; ++q) {
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) {
+                        s[i] = s[i + 1];
+                    }
+                    s[7] = col_ptr[(long long)min(cy + 8 + q, height - 1) * pitch];
+                }
+                cy += per_pass;
+            }
+            float d;
+            if (!EXACT) {
+                const float col = col_ssd<CM>(c, s);
+                const float p1 = col + __shfl_down(col, 1);
+                const float p2 = p1 + __shfl_down(p1, 2);
+                d = p2 + __shfl_down(p2, 4);
+            } else {
+                float S[8];
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) {
+                    S[cc] = col_ssd<CM>(cur_reg + cc * 8, s);
+                }
+                const float t01 = S[0] + __shfl_down(S[1], 1);
+                const float t23 = __shfl_down(S[2], 2) + __shfl_down(S[3], 3);
+                const float t45 = __shfl_down(S[4], 4) + __shfl_down(S[5], 5);
+                const float t67 = __shfl_down(S[6], 6) + __shfl_down(S[7], 7);
+                d = (t01 + t23) + (t45 + t67);
+            }
+            const int row = cy - top;
+            const bool cand = lane_on && kk < nc && row < rh;
+            const int seq = seq_base + row * rw + j;
+            const uint32_t xy = uint32_t(left + j) | (uint32_t(cy) << 16);
+            list_offer(L, cand, d, seq, xy, lane);
+        }
+    }
+}
+
+// Group list (spatial top-8 plus temporal inserts) held by lanes 0..7.
+struct GroupList {
+    float e;
+    uint32_t xy;
+    int z;
+};
+
+// ginsert8 of the shader: strict '<' on the distance, after equal ones
+__device__ __forceinline__ void group_insert(GroupList &G, float fe, uint32_t fxy, int z, int lane)
+{
+    const float e7 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(G.e), 7));
+    if (!(fe < e7)) {
+        return;
+    }
+    const uint64_t before = __ballot(lane < 8 && G.e <= fe);
+    const int pos = __popcll(before & 0xFFull);
+    const float up_e = __shfl_up(G.e, 1);
+    const uint32_t up_xy = uint32_t(__shfl_up(int(G.xy), 1));
+    const int up_z = __shfl_up(G.z, 1);
+    if (lane == pos) {
+        G.e = fe;
+        G.xy = fxy;
+        G.z = z;
+    } else if (lane > pos && lane < 8) {
+        G.e = up_e;
+        G.xy = up_xy;
+        G.z = up_z;
+    }
+}
+
+template <int CM, bool EXACT>
+__global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
+{
+    __shared__ float cur_all[4][64];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+
+    // XCD-aware order: consecutive workgroups go to different XCDs, so give
+    // every XCD a contiguous band of the frame (its L2 then holds the band)
+    int bid = blockIdx.x;
+    const int nwg = gridDim.x;
+    const int nfull = (nwg / 8) * 8;
+    if (bid < nfull) {
+        bid = (bid % 8) * (nwg / 8) + bid / 8;
+    }
+    const int blk = bid * 4 + wave;
+    const int nref = A.nx * A.ny;
+    if (blk >= nref) {
+        return;  // whole wavefront; no block-wide barrier follows
+    }
+    const int by = blk / A.nx;
+    const int bx = blk - by * A.nx;
+    const int W = A.width, H = A.height, pitch = A.pitch, r = A.radius;
+    const int x = min(bx * A.step, W - 8);
+    const int y = min(by * A.step, H - 8);
+
+    const float *centre = A.plane[r];
+    float *cur_lds = cur_all[wave];
+    // cur[c*8+i] = pixel (x+c, y+i), as the shader's cur[] (src/bm3d.comp:460-466)
+    cur_lds[lane] = centre[(long long)(y + (lane & 7)) * pitch + x + (lane >> 3)];
+    wave_lds_sync();
+    float cur_reg[EXACT ? 64 : 1];
+    if (EXACT) {
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+            cur_reg[i] = cur_lds[i];
+        }
+    }
+
+    // spatial search (src/bm3d.comp:479-538)
+    TopList L;
+    list_reset(L);
+    {
+        const int R = A.bm_range;
+        const int left = max(x - R, 0), right = min(x + R, W - 8);
+        const int top = max(y - R, 0), bottom = min(y + R, H - 8);
+        scan_window<CM, EXACT>(L, centre, pitch, H, cur_lds, cur_reg, left, right, top, bottom, 0, lane);
+    }
+    GroupList G;
+    G.e = __uint_as_float(uint32_t(L.key >> 32));
+    G.xy = L.xy;
+    G.z = r;
+
+    // temporal predictive search (src/bm3d.comp:540-637)
+    if (r > 0) {
+        const uint32_t spatial_xy = L.xy;
+        const int PR = A.ps_range;
+        const int win = (2 * PR + 1) * (2 * PR + 1);
+        for (int dir = -1; dir <= 1; dir += 2) {
+            uint32_t pred = spatial_xy;
+            for (int t = 1; t <= r; ++t) {
+                const int z = r + dir * t;
+                const float *plane = A.plane[z];
+                TopList F;
+                list_reset(F);
+                for (int i = 0; i < A.ps_num; ++i) {
+                    const uint32_t pxy = readlane_u32(pred, i);
+                    const int cx = int(pxy & 0xFFFFu), cyy = int(pxy >> 16);
+                    const int left = max(cx - PR, 0), right = min(cx + PR, W - 8);
+                    const int top = max(cyy - PR, 0), bottom = min(cyy + PR, H - 8);
+                    scan_window<CM, EXACT>(F, plane, pitch, H, cur_lds, cur_reg, left, right, top, bottom,
+                                           i * win, lane);
+                }
+                for (int i = 0; i < A.ps_num; ++i) {
+                    const uint64_t fk = readlane_u64(F.key, i);
+                    const uint32_t fxy = readlane_u32(F.xy, i);
+                    group_insert(G, __uint_as_float(uint32_t(fk >> 32)), fxy, z, lane);
+                }
+                pred = F.xy;
+            }
+        }
+    }
+
+    // the reference block itself is always part of the group (src/bm3d.comp:639-663)
+    const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
+    const uint64_t present = __ballot(lane < 8 && G.xy == self_xy && G.z == r);
+    if (present == 0) {
+        const float up_e = __shfl_up(G.e, 1);
+        const uint32_t up_xy = uint32_t(__shfl_up(int(G.xy), 1));
+        const int up_z = __shfl_up(G.z, 1);
+        if (lane == 0) {
+            G.e = -1.0f;
+            G.xy = self_xy;
+            G.z = r;
+        } else if (lane < 8) {
+            G.e = up_e;
+            G.xy = up_xy;
+            G.z = up_z;
+        }
+    }
+    if (lane < 8) {
+        const uint32_t gx = G.xy & 0xFFFFu, gy = G.xy >> 16;
+        A.groups[(long long)blk * 8 + lane] = pack_entry(gx, gy, uint32_t(G.z));
+        if (A.dists) {
+            A.dists[(long long)blk * 8 + lane] = G.e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 8-point DCT butterflies.  Forward: 4 x orthonormal DCT-II, inverse:
+// 4 x orthonormal DCT-III, with the operation sequence of the reference's
+// dct_fwd / dct_inv (src/bm3d.comp:91-181) so every rounding matches.
+// ---------------------------------------------------------------------------
+template <int CM>
+__device__ __forceinline__ float fma_x(float a, float b, float c)
+{
+    if (CM >= kFpExplicit) {
+        return __builtin_fmaf(a, b, c);
+    }
+    const float t = a * b;
+    return t + c;
+}
+
+template <int CM>
+__device__ __forceinline__ float fma_c(float a, float b, float c)
+{
+    if (CM >= kFpAll) {
+        return __builtin_fmaf(a, b, c);
+    }
+    const float t = a * b;
+    return t + c;
+}
+
+constexpr float kTanPi8 = 0.414213562373095048801688724209698078569671875f;
+constexpr float k2CosPi8 = 1.847759065022573512256366378793576573644833252f;
+constexpr float kTanPi16 = 0.198912367379658006911597622644676228597850501f;
+constexpr float k2CosPi16 = 1.961570560806460898252364472268478073947867462f;
+constexpr float kSqrt2 = 1.414213562373095048801688724209698078569671875f;
+constexpr float kTan3Pi16 = 0.668178637919298919997757686523080761552472251f;
+constexpr float k2Cos3Pi16 = 1.662939224605090474157576755235811513477121624f;
+constexpr float kInvSqrt2 = 0.707106781186547524400844362104849039284835938f;
+
+// v[i * st] for i = 0..7
+template <int CM, int ST>
+__device__ __forceinline__ void dct8_fwd(float *v)
+{
+    const float d07 = v[0] - v[7 * ST], s07 = v[0] + v[7 * ST];
+    const float d43 = v[4 * ST] - v[3 * ST], s43 = v[4 * ST] + v[3 * ST];
+    const float d25 = v[2 * ST] - v[5 * ST], s25 = v[2 * ST] + v[5 * ST];
+    const float d16 = v[1 * ST] - v[6 * ST], s16 = v[1 * ST] + v[6 * ST];
+    const float a = d25 + d16;
+    const float f = d25 - d16;
+    const float b = fma_x<CM>(-kInvSqrt2, a, d07);
+    const float g = fma_x<CM>(-kInvSqrt2, f, d43);
+    const float o3 = k2Cos3Pi16 * fma_x<CM>(kTan3Pi16, g, b);
+    const float o5 = -(k2Cos3Pi16 * fma_x<CM>(-kTan3Pi16, b, g));
+    const float p = s07 + s43;
+    const float q = s25 + s16;
+    const float o4 = kSqrt2 * (p - q);
+    const float o0 = kSqrt2 * (p + q);
+    const float h = fma_x<CM>(kInvSqrt2, a, d07);
+    const float i = fma_x<CM>(kInvSqrt2, f, d43);
+    const float o1 = k2CosPi16 * fma_x<CM>(-kTanPi16, i, h);
+    const float o7 = k2CosPi16 * fma_x<CM>(kTanPi16, h, i);
+    const float l = s07 - s43;
+    const float o = s25 - s16;
+    const float o2 = k2CosPi8 * fma_x<CM>(-kTanPi8, o, l);
+    const float o6 = k2CosPi8 * fma_x<CM>(kTanPi8, l, o);
+    v[0] = o0;
+    v[1 * ST] = o1;
+    v[2 * ST] = o2;
+    v[3 * ST] = o3;
+    v[4 * ST] = o4;
+    v[5 * ST] = o5;
+    v[6 * ST] = o6;
+    v[7 * ST] = o7;
+}
+
+template <int CM, int ST>
+__device__ __forceinline__ void dct8_inv(float *v)
+{
+    const float dc = v[0] * kSqrt2;
+    const float e_plus = fma_x<CM>(kSqrt2, v[4 * ST], dc);
+    const float e_minus = fma_x<CM>(-kSqrt2, v[4 * ST], dc);
+    const float r26 = fma_x<CM>(kTanPi8, v[6 * ST], v[2 * ST]);
+    const float r62 = fma_x<CM>(kTanPi8, v[2 * ST], -v[6 * ST]);
+    const float s53 = v[5 * ST] + v[3 * ST];
+    const float d35 = v[3 * ST] - v[5 * ST];
+    const float c1 = fma_x<CM>(kInvSqrt2, s53, v[1 * ST]);
+    const float n7 = fma_x<CM>(-kInvSqrt2, d35, v[7 * ST]);
+    const float f7 = fma_x<CM>(kInvSqrt2, d35, v[7 * ST]);
+    const float m1 = fma_x<CM>(-kInvSqrt2, s53, v[1 * ST]);
+    const float t7 = fma_x<CM>(k2CosPi8, r26, e_plus);
+    const float g = fma_x<CM>(kTanPi16, f7, c1);
+    const float o7 = fma_x<CM>(-k2CosPi16, g, t7);
+    const float o0 = fma_x<CM>(k2CosPi16, g, t7);
+    const float tp = fma_x<CM>(-k2CosPi8, r62, e_minus);
+    const float tq = fma_x<CM>(kTan3Pi16, m1, n7);
+    const float o5 = fma_x<CM>(-k2Cos3Pi16, tq, tp);
+    const float o2 = fma_x<CM>(k2Cos3Pi16, tq, tp);
+    const float th = fma_x<CM>(-k2CosPi8, r26, e_plus);
+    const float ti = fma_x<CM>(-kTanPi16, c1, f7);
+    const float o3 = fma_x<CM>(-k2CosPi16, ti, th);
+    const float o4 = fma_x<CM>(k2CosPi16, ti, th);
+    const float tl = fma_x<CM>(k2CosPi8, r62, e_minus);
+    const float to = fma_x<CM>(-kTan3Pi16, n7, m1);
+    const float o6 = fma_x<CM>(-k2Cos3Pi16, to, tl);
+    const float o1 = fma_x<CM>(k2Cos3Pi16, to, tl);
+    v[0] = o0;
+    v[1 * ST] = o1;
+    v[2 * ST] = o2;
+    v[3 * ST] = o3;
+    v[4 * ST] = o4;
+    v[5 * ST] = o5;
+    v[6 * ST] = o6;
+    v[7 * ST] = o7;
+}
+
+// transform along the index inside each block of 8 (d[it*8 + i] over i)
+template <int CM, bool INV>
+__device__ __forceinline__ void transform_rows(float *d)
+{
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        if (INV) {
+            dct8_inv<CM, 1>(d + it * 8);
+        } else {
+            dct8_fwd<CM, 1>(d + it * 8);
+        }
+    }
+}
+
+// transform along the block index (d[it + i*8] over i)
+template <int CM, bool INV>
+__device__ __forceinline__ void transform_blocks(float *d)
+{
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        if (INV) {
+            dct8_inv<CM, 8>(d + it);
+        } else {
+            dct8_fwd<CM, 8>(d + it);
+        }
+    }
+}
+
+// 8x8 transpose of every block d[it*8 + i] across the 8 lanes of a group:
+// afterwards lane c holds what lane i held at index c.  tb is the group's
+// 8 x 9 LDS tile.
+__device__ __forceinline__ void transpose_group(float *d, float *tb, int c)
+{
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            tb[i * 9 + c] = d[it * 8 + i];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            d[it * 8 + i] = tb[c * 9 + i];
+        }
+        wave_lds_sync();
+    }
+}
+
+__device__ __forceinline__ float group8_sum_tree(float k)
+{
+    // ((k0+k1)+(k2+k3))+((k4+k5)+(k6+k7)) over the 8 lanes of a group
+    k = k + __shfl_xor(k, 1);
+    k = k + __shfl_xor(k, 2);
+    k = k + __shfl_xor(k, 4);
+    return k;
+}
+
+template <int CM, bool FINAL>
+__global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
+{
+    __shared__ float tile[4][8][72];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 3;  // group within the wavefront
+    const int c = lane & 7;   // column inside the block
+    float *tb = tile[wave][g];
+
+    int bid = blockIdx.x;
+    const int nwg = gridDim.x;
+    const int nfull = (nwg / 8) * 8;
+    if (bid < nfull) {
+        bid = (bid % 8) * (nwg / 8) + bid / 8;
+    }
+    const int gid = (bid * 4 + wave) * 8 + g;
+    const bool active = gid < A.ngroups;
+    const int gsafe = active ? gid : 0;
+
+    uint32_t ent[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        ent[i] = A.groups[(long long)gsafe * 8 + i];
+    }
+
+    // gather: d[i*8 + j] = block i, row j, column c (src/bm3d.comp:677-689)
+    float d[64];
+    float rf[FINAL ? 64 : 1];
+    const long long pitch = A.pitch;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const long long off = (long long)entry_y(ent[i]) * pitch + entry_x(ent[i]) + c;
+        const float *base = A.input[entry_z(ent[i])] + off;
+        const float *rbase = A.search[entry_z(ent[i])] + off;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            d[i * 8 + j] = base[j * pitch];
+            if (FINAL) {
+                rf[i * 8 + j] = rbase[j * pitch];
+            }
+        }
+    }
+
+    // forward: vertical, horizontal, group axis (collaborative_hard / _wiener)
+    transform_rows<CM, false>(d);
+    transpose_group(d, tb, c);
+    transform_rows<CM, false>(d);
+    transpose_group(d, tb, c);
+    transform_blocks<CM, false>(d);
+    if (FINAL) {
+        transform_rows<CM, false>(rf);
+        transpose_group(rf, tb, c);
+        transform_rows<CM, false>(rf);
+        transpose_group(rf, tb, c);
+        transform_blocks<CM, false>(rf);
+    }
+
+    // lane c now holds column frequency u = c, d[w*8 + v]
+    const float scale = 1.0f / 4096.0f;
+    float ks[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+        const float val = d[i];
+        if (!FINAL) {
+            // hard threshold, the DC term always kept (src/bm3d.comp:358-382)
+            const float thr = (i == 0 && c == 0) ? 0.0f : A.sigma;
+            const bool keep = fabsf(val) >= thr;
+            ks[i % 4] += keep ? 1.0f : 0.0f;
+            d[i] = keep ? val * scale : 0.0f;
+        } else {
+            // Wiener shrinkage against the reference's spectrum (src/bm3d.comp:403-420)
+            const float rv = rf[i];
+            const float r2 = rv * rv;
+            const float s2 = A.sigma * A.sigma;
+            float coeff = r2 / fma_c<CM>(rv, rv, s2);
+            if (i == 0 && c == 0) {
+                coeff = 1.0f;
+            }
+            const float filtered = val * coeff;
+            ks[i % 4] = fma_c<CM>(coeff, coeff, ks[i % 4]);
+            d[i] = filtered * scale;
+        }
+    }
+    const float kl = (ks[0] + ks[1]) + (ks[2] + ks[3]);
+    const float k = group8_sum_tree(kl);
+    const float w = 1.0f / k;
+
+    // inverse: vertical, horizontal, group axis
+    transform_rows<CM, true>(d);
+    transpose_group(d, tb, c);
+    transform_rows<CM, true>(d);
+    transpose_group(d, tb, c);
+    transform_blocks<CM, true>(d);
+
+    // weighted scatter (src/bm3d.comp:697-715)
+    if (active) {
+        const float E = A.extractor;
+        const float wq = (w + E) - E;
+        const long long rp = A.res_pitch;
+        const long long pe = (long long)A.height * rp;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            float *wd = A.res + (long long)entry_z(ent[i]) * 2 * pe + (long long)entry_y(ent[i]) * rp +
+                        entry_x(ent[i]) + c;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float v = fma_c<CM>(w, d[i * 8 + j], E) - E;
+                unsafeAtomicAdd(wd + j * rp, v);
+                unsafeAtomicAdd(wd + pe + j * rp, wq);
+            }
+        }
+    }
+}
+
+// dst = float(sum double(wdst) / sum double(weight)) over the tw slices
+// (src/bm3d_agg.comp:36-51)
+__global__ __launch_bounds__(256) void bm3d_aggregate(AggArgs A)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= A.width || y >= A.height) {
+        return;
+    }
+    const long long q = (long long)y * A.pitch + x;
+    double acc = 0.0, acw = 0.0;
+    for (int i = 0; i < A.tw; ++i) {
+        acc += (double)A.slice[i][q];
+        acw += (double)A.slice[i][q + A.plane_elems];
+    }
+    A.dst[(long long)y * A.dst_pitch + x] = (float)(acc / acw);
+}
+
+__global__ __launch_bounds__(256) void fill_zero(float4 *p, long long n4)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
+{
+    const int nref = a.nx * a.ny;
+    const dim3 grid((nref + 3) / 4), block(256);
+    const bool exact = a.radius >= 3;
+    const bool fused = fp_mode == kFpAll;
+    if (exact) {
+        if (fused) {
+            hipLaunchKernelGGL((bm3d_match<kFpAll, true>), grid, block, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((bm3d_match<kFpExplicit, true>), grid, block, 0, s, a);
+        }
+    } else {
+        if (fused) {
+            hipLaunchKernelGGL((bm3d_match<kFpAll, false>), grid, block, 0, s, a);
+        } else {
+            hipLaunchKernelGGL((bm3d_match<kFpExplicit, false>), grid, block, 0, s, a);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, hipStream_t s)
+{
+    const dim3 grid((a.ngroups + 31) / 32), block(256);
+#define VSFEEL_EST(CM)                                                             \
+    if (final_pass) {                                                              \
+        hipLaunchKernelGGL((bm3d_estimate<CM, true>), grid, block, 0, s, a);       \
+    } else {                                                                       \
+        hipLaunchKernelGGL((bm3d_estimate<CM, false>), grid, block, 0, s, a);      \
+    }
+    if (fp_mode == kFpNone) {
+        VSFEEL_EST(kFpNone)
+    } else if (fp_mode == kFpAll) {
+        VSFEEL_EST(kFpAll)
+    } else {
+        VSFEEL_EST(kFpExplicit)
+    }
+#undef VSFEEL_EST
+    return hipGetLastError();
+}
+
+hipError_t launch_aggregate(const AggArgs &a, hipStream_t s)
+{
+    const dim3 grid((a.width + 63) / 64, (a.height + 3) / 4), block(256);
+    hipLaunchKernelGGL(bm3d_aggregate, grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float *p, long long n, hipStream_t s)
+{
+    // n is a multiple of 4 (planes are padded to 64 floats)
+    const long long n4 = n / 4;
+    long long blocks = (n4 + 255) / 256;
+    if (blocks > 4096) {
+        blocks = 4096;
+    }
+    if (blocks < 1) {
+        blocks = 1;
+    }
+    hipLaunchKernelGGL(fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<float4 *>(p), n4);
+    return hipGetLastError();
+}
+
+}  // namespace vsfeel
