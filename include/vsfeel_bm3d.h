@@ -174,8 +174,8 @@ int vsfeel_bm3d_debug_estimate(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
  * events are recorded around every launch on the launch stream.
  * vsfeel_bm3d_set_timing resets the counters; vsfeel_bm3d_kernel_times waits
  * for the recorded launches and returns the summed milliseconds and launch
- * counts per kernel: [0] slot clear, [1] block matching, [2] collaborative
- * filtering + scatter, [3] aggregation.
+ * counts per kernel (arrays of 5): [0] slot clear, [1] block matching,
+ * [2] collaborative filtering + scatter, [3] aggregation, [4] tile reduction.
  */
 int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
 int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);

@@ -46,6 +46,8 @@ int32_t bm3d_oracle_agg_z(int32_t i, int32_t n, int32_t nframes, int32_t radius)
 
 #define REAL float
 #define RES_T float
+/* the shader's float constants (decimal -> float) */
+#define RLIT(x) (x##f)
 #define RFMA fmaf
 #define FABS fabsf
 #define FN(name) bm3d_oracle_f32_##name
@@ -55,9 +57,12 @@ int32_t bm3d_oracle_agg_z(int32_t i, int32_t n, int32_t nframes, int32_t radius)
 #undef RFMA
 #undef FABS
 #undef FN
+#undef RLIT
 
 #define REAL double
 #define RES_T double
+/* exact double constants: the float64 oracle is the algorithm without float rounding */
+#define RLIT(x) (x)
 #define RFMA fma
 #define FABS fabs
 #define FN(name) bm3d_oracle_f64_##name
@@ -67,3 +72,4 @@ int32_t bm3d_oracle_agg_z(int32_t i, int32_t n, int32_t nframes, int32_t radius)
 #undef RFMA
 #undef FABS
 #undef FN
+#undef RLIT

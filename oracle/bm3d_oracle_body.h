@@ -35,14 +35,14 @@ static inline REAL FN(fma_c)(REAL a, REAL b, REAL c, int mode)
  */
 static void FN(dct8_fwd)(REAL *v, int mode)
 {
-    const REAL c_tan_pi_8 = (REAL)0.414213562373095048801688724209698078569671875f;
-    const REAL c_2cos_pi_8 = (REAL)1.847759065022573512256366378793576573644833252f;
-    const REAL c_tan_pi_16 = (REAL)0.198912367379658006911597622644676228597850501f;
-    const REAL c_2cos_pi_16 = (REAL)1.961570560806460898252364472268478073947867462f;
-    const REAL c_sqrt2 = (REAL)1.414213562373095048801688724209698078569671875f;
-    const REAL c_tan_3pi_16 = (REAL)0.668178637919298919997757686523080761552472251f;
-    const REAL c_2cos_3pi_16 = (REAL)1.662939224605090474157576755235811513477121624f;
-    const REAL c_inv_sqrt2 = (REAL)0.707106781186547524400844362104849039284835938f;
+    const REAL c_tan_pi_8 = RLIT(0.414213562373095048801688724209698078569671875);
+    const REAL c_2cos_pi_8 = RLIT(1.847759065022573512256366378793576573644833252);
+    const REAL c_tan_pi_16 = RLIT(0.198912367379658006911597622644676228597850501);
+    const REAL c_2cos_pi_16 = RLIT(1.961570560806460898252364472268478073947867462);
+    const REAL c_sqrt2 = RLIT(1.414213562373095048801688724209698078569671875);
+    const REAL c_tan_3pi_16 = RLIT(0.668178637919298919997757686523080761552472251);
+    const REAL c_2cos_3pi_16 = RLIT(1.662939224605090474157576755235811513477121624);
+    const REAL c_inv_sqrt2 = RLIT(0.707106781186547524400844362104849039284835938);
 
     const REAL d07 = v[0] - v[7], s07 = v[0] + v[7];
     const REAL d43 = v[4] - v[3], s43 = v[4] + v[3];
@@ -73,14 +73,14 @@ static void FN(dct8_fwd)(REAL *v, int mode)
 /* 8-point inverse butterfly of src/bm3d.comp:137-181 (4 * orthonormal DCT-III) */
 static void FN(dct8_inv)(REAL *v, int mode)
 {
-    const REAL c_2cos_3pi_16 = (REAL)1.662939224605090474157576755235811513477121624f;
-    const REAL c_tan_3pi_16 = (REAL)0.668178637919298919997757686523080761552472251f;
-    const REAL c_2cos_pi_16 = (REAL)1.961570560806460898252364472268478073947867462f;
-    const REAL c_tan_pi_16 = (REAL)0.198912367379658006911597622644676228597850501f;
-    const REAL c_2cos_pi_8 = (REAL)1.847759065022573512256366378793576573644833252f;
-    const REAL c_inv_sqrt2 = (REAL)0.707106781186547524400844362104849039284835938f;
-    const REAL c_tan_pi_8 = (REAL)0.414213562373095048801688724209698078569671875f;
-    const REAL c_sqrt2 = (REAL)1.414213562373095048801688724209698078569671875f;
+    const REAL c_2cos_3pi_16 = RLIT(1.662939224605090474157576755235811513477121624);
+    const REAL c_tan_3pi_16 = RLIT(0.668178637919298919997757686523080761552472251);
+    const REAL c_2cos_pi_16 = RLIT(1.961570560806460898252364472268478073947867462);
+    const REAL c_tan_pi_16 = RLIT(0.198912367379658006911597622644676228597850501);
+    const REAL c_2cos_pi_8 = RLIT(1.847759065022573512256366378793576573644833252);
+    const REAL c_inv_sqrt2 = RLIT(0.707106781186547524400844362104849039284835938);
+    const REAL c_tan_pi_8 = RLIT(0.414213562373095048801688724209698078569671875);
+    const REAL c_sqrt2 = RLIT(1.414213562373095048801688724209698078569671875);
 
     const REAL dc = v[0] * c_sqrt2;
     const REAL e_plus = FN(fma_x)(c_sqrt2, v[4], dc, mode);

@@ -36,11 +36,45 @@ struct EstimateArgs {
     const float *input[kMaxWindow];  // clip being denoised (== search outside the final pass)
     const uint32_t *groups;
     float *res;                      // [2r+1][2][height*res_pitch] accumulators
-    int height, pitch;               // pitch of the search / input planes
+    int width, height, pitch;        // pitch of the search / input planes
     int res_pitch;
     int ngroups;
     float sigma;
     float extractor;
+    int radius;
+    // tiled accumulation (scratch != nullptr): tiles of tile_bx x tile_by ref
+    // blocks, each accumulating its own frame's contributions over a region of
+    // at most region_w x region_h pixels
+    float *scratch;                  // [tiles][2][region_h][region_w]
+    int nx, ny, step, halo;
+    int tile_bx, tile_by, tiles_x, tiles_y;
+    int region_w, region_h;
+};
+
+// Pixel span [lo, hi] that the groups of tile t (along one axis) can reach in
+// their own frame: the tile's ref blocks, widened by the search range.
+__host__ __device__ inline void tile_span(int t, int tile_blocks, int nblocks, int step, int size, int halo,
+                                          int *lo, int *hi)
+{
+    const int b0 = t * tile_blocks;
+    const int b1 = (b0 + tile_blocks < nblocks ? b0 + tile_blocks : nblocks) - 1;
+    const int p0 = b0 * step < size - 8 ? b0 * step : size - 8;
+    const int p1 = b1 * step < size - 8 ? b1 * step : size - 8;
+    *lo = p0 - halo > 0 ? p0 - halo : 0;
+    *hi = (p1 + halo < size - 8 ? p1 + halo : size - 8) + 7;
+}
+
+// Reduction of the tile accumulators into the dense slice of the frame itself.
+struct ReduceArgs {
+    const float *scratch;
+    float *res;                      // wdst plane of the slice; the weight plane follows at +height*res_pitch
+    const int2 *col_tiles;           // per column: first and last tile covering it
+    const int2 *row_tiles;           // per row
+    const int *tile_x0;              // per tile column: region origin
+    const int *tile_y0;              // per tile row
+    int width, height, res_pitch;
+    int tiles_x, region_w, region_h;
+    int add_existing;                // add to the slice instead of overwriting it
 };
 
 // Aggregation of the 2r+1 slices that land on one output frame.

@@ -24,6 +24,7 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s);
 hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, hipStream_t s);
 hipError_t launch_aggregate(const AggArgs &a, hipStream_t s);
 hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
+hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s);
 
 namespace {
 
@@ -56,7 +57,43 @@ int agg_z(int i, int n, int nframes, int radius)
     return std::min(std::max(2 * radius - i, n - nframes + 1 + radius), n + radius);
 }
 
-enum KernelKind { kKernelFill = 0, kKernelMatch, kKernelEstimate, kKernelAggregate, kKernelKinds };
+enum KernelKind { kKernelFill = 0, kKernelMatch, kKernelEstimate, kKernelAggregate, kKernelReduce, kKernelKinds };
+
+// LDS bytes a tile accumulator may take (one 512-thread workgroup per CU,
+// 16 bytes per region pixel), and the groups one pass of it filters
+constexpr int kTileLdsBudget = 112 * 1024;
+constexpr int kTileGroupsPerPass = 64;
+
+// Tile shape of the tiled estimate: the most ref blocks per workgroup whose
+// accumulator fits the LDS budget, preferring whole passes of 32 groups.
+bool choose_tiles(int width, int height, int step, int range, int *tbx, int *tby, int *rw, int *rh)
+{
+    double best = 1e30;
+    bool found = false;
+    for (int by = 1; by <= 32; ++by) {
+        for (int bx = 1; bx <= 32; ++bx) {
+            const int w = std::min((bx - 1) * step + 8 + 2 * range, width);
+            const int h = std::min((by - 1) * step + 8 + 2 * range, height);
+            if (2LL * w * h * 8 > kTileLdsBudget) {
+                continue;
+            }
+            const int groups = bx * by;
+            const int passes = (groups + kTileGroupsPerPass - 1) / kTileGroupsPerPass;
+            // idle lanes per group plus a small charge for the scratch traffic
+            const double cost = double(passes * kTileGroupsPerPass) / groups +
+                                0.05 * double(w) * h / (double(bx * step) * (by * step));
+            if (cost < best - 1e-12) {
+                best = cost;
+                *tbx = bx;
+                *tby = by;
+                *rw = w;
+                *rh = h;
+                found = true;
+            }
+        }
+    }
+    return found;
+}
 
 }  // namespace
 
@@ -77,6 +114,13 @@ struct Instance {
     uint32_t *groups = nullptr;  // nref * 8
     float *dists = nullptr;      // nref * 8 (debug)
     float *dst_dev = nullptr;    // one plane
+
+    // tiled accumulation of the frame's own slice (see bm3d_estimate_tiled)
+    bool tiled = false;
+    bool short_windows = false;  // some spatial window has fewer than 8 candidates
+    int tile_bx = 0, tile_by = 0, tiles_x = 0, tiles_y = 0, region_w = 0, region_h = 0;
+    float *scratch = nullptr;    // tiles * 2 * region_h * region_w
+    int *tile_tables = nullptr;  // col_tiles[W] (int2), row_tiles[H] (int2), tile_x0[tiles_x], tile_y0[tiles_y]
     hipStream_t stream = nullptr;
     hipEvent_t last_done = nullptr;  // end of the previous device-side call
     bool have_last = false;
@@ -174,6 +218,8 @@ struct Instance {
         (void)hipFree(groups);
         (void)hipFree(dists);
         (void)hipFree(dst_dev);
+        (void)hipFree(scratch);
+        (void)hipFree(tile_tables);
         if (last_done) {
             (void)hipEventDestroy(last_done);
         }
@@ -196,11 +242,16 @@ struct Instance {
     hipError_t estimate(const float *const *search, const float *const *input, int in_pitch,
                         float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr)
     {
-        hipError_t e = timed_launch(kKernelFill, s, [&] {
-            return launch_fill_zero(stack, (long long)tw * 2 * height * stack_pitch, s);
-        });
-        if (e != hipSuccess) {
-            return e;
+        hipError_t e = hipSuccess;
+        if (!tiled || radius > 0 || short_windows) {
+            // the tile reduction overwrites the frame's own slice; the other
+            // slices are accumulated with atomics and must start at zero
+            e = timed_launch(kKernelFill, s, [&] {
+                return launch_fill_zero(stack, (long long)tw * 2 * height * stack_pitch, s);
+            });
+            if (e != hipSuccess) {
+                return e;
+            }
         }
         MatchArgs m {};
         for (int z = 0; z < tw; ++z) {
@@ -235,7 +286,40 @@ struct Instance {
         a.ngroups = nref;
         a.sigma = sigma;
         a.extractor = extractor;
-        return timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, s); });
+        a.width = width;
+        a.radius = radius;
+        a.nx = nx;
+        a.ny = ny;
+        a.step = block_step;
+        a.halo = bm_range;
+        if (tiled) {
+            a.scratch = scratch;
+            a.tile_bx = tile_bx;
+            a.tile_by = tile_by;
+            a.tiles_x = tiles_x;
+            a.tiles_y = tiles_y;
+            a.region_w = region_w;
+            a.region_h = region_h;
+        }
+        e = timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, s); });
+        if (e != hipSuccess || !tiled) {
+            return e;
+        }
+        ReduceArgs ra {};
+        ra.scratch = scratch;
+        ra.res = stack + (long long)radius * 2 * height * stack_pitch;
+        ra.col_tiles = reinterpret_cast<const int2 *>(tile_tables);
+        ra.row_tiles = reinterpret_cast<const int2 *>(tile_tables + 2 * width);
+        ra.tile_x0 = tile_tables + 2 * width + 2 * height;
+        ra.tile_y0 = ra.tile_x0 + tiles_x;
+        ra.width = width;
+        ra.height = height;
+        ra.res_pitch = stack_pitch;
+        ra.tiles_x = tiles_x;
+        ra.region_w = region_w;
+        ra.region_h = region_h;
+        ra.add_existing = short_windows ? 1 : 0;
+        return timed_launch(kKernelReduce, s, [&] { return launch_tile_reduce(ra, s); });
     }
 
     // aggregation of frame n from the res slots (record_bm3d_agg, src/bm3d.cpp:628-696)
@@ -433,6 +517,18 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
     d.nref = d.nx * d.ny;
     d.pitch = (d.width + 63) & ~63;
     d.pe = (long long)d.height * d.pitch;
+    {
+        // the smallest spatial window sits at a corner of the frame
+        const int rw = std::min(d.bm_range, d.width - 8) + 1;
+        const int rh = std::min(d.bm_range, d.height - 8) + 1;
+        d.short_windows = (long long)rw * rh < 8;
+    }
+    d.tiled = choose_tiles(d.width, d.height, d.block_step, d.bm_range, &d.tile_bx, &d.tile_by, &d.region_w,
+                           &d.region_h);
+    if (d.tiled) {
+        d.tiles_x = (d.nx + d.tile_bx - 1) / d.tile_bx;
+        d.tiles_y = (d.ny + d.tile_by - 1) / d.tile_by;
+    }
     return "";
 }
 
@@ -494,6 +590,45 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         (e = hipMalloc(&d.dists, size_t(d.nref) * 8 * sizeof(float))) != hipSuccess ||
         (e = hipMalloc(&d.dst_dev, plane)) != hipSuccess) {
         return dev_fail(e, "hipMalloc");
+    }
+    if (d.tiled) {
+        // which tiles cover each column / row, and every tile's region origin
+        std::vector<int> tab(size_t(2) * d.width + 2 * d.height + d.tiles_x + d.tiles_y);
+        std::vector<int> x_lo(d.tiles_x), x_hi(d.tiles_x), y_lo(d.tiles_y), y_hi(d.tiles_y);
+        for (int t = 0; t < d.tiles_x; ++t) {
+            tile_span(t, d.tile_bx, d.nx, d.block_step, d.width, d.bm_range, &x_lo[t], &x_hi[t]);
+        }
+        for (int t = 0; t < d.tiles_y; ++t) {
+            tile_span(t, d.tile_by, d.ny, d.block_step, d.height, d.bm_range, &y_lo[t], &y_hi[t]);
+        }
+        auto ranges = [](int n, const std::vector<int> &lo, const std::vector<int> &hi, int *out) {
+            for (int p = 0; p < n; ++p) {
+                int first = -1, last = -2;
+                for (size_t t = 0; t < lo.size(); ++t) {
+                    if (lo[t] <= p && p <= hi[t]) {
+                        if (first < 0) {
+                            first = int(t);
+                        }
+                        last = int(t);
+                    }
+                }
+                out[2 * p] = first < 0 ? 0 : first;
+                out[2 * p + 1] = last;  // an uncovered pixel gets an empty range
+            }
+        };
+        ranges(d.width, x_lo, x_hi, tab.data());
+        ranges(d.height, y_lo, y_hi, tab.data() + 2 * d.width);
+        std::copy(x_lo.begin(), x_lo.end(), tab.begin() + 2 * d.width + 2 * d.height);
+        std::copy(y_lo.begin(), y_lo.end(), tab.begin() + 2 * d.width + 2 * d.height + d.tiles_x);
+        const size_t scratch_elems = size_t(d.tiles_x) * d.tiles_y * 2 * d.region_h * d.region_w;
+        if ((e = hipMalloc(&d.scratch, scratch_elems * sizeof(float))) != hipSuccess ||
+            (e = hipMalloc(&d.tile_tables, tab.size() * sizeof(int))) != hipSuccess) {
+            return dev_fail(e, "hipMalloc");
+        }
+        if ((e = hipMemcpy(d.tile_tables, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+            return dev_fail(e, "hipMemcpy");
+        }
     }
     d.src_frame.assign(d.src_ring, -1);
     d.res_frame.assign(d.res_cap, -1);

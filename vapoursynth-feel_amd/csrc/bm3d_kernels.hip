@@ -583,34 +583,30 @@ __device__ __forceinline__ float group8_sum_tree(float k)
     return k;
 }
 
-template <int CM, bool FINAL>
-__global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
+// XCD-aware workgroup order: the hardware hands consecutive workgroups to
+// the 8 XCDs in turn, so give every XCD a contiguous run of the work (its L2
+// then holds one band of the frame)
+__device__ __forceinline__ int xcd_block_id()
 {
-    __shared__ float tile[4][8][72];
-    const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
-    const int g = lane >> 3;  // group within the wavefront
-    const int c = lane & 7;   // column inside the block
-    float *tb = tile[wave][g];
-
     int bid = blockIdx.x;
     const int nwg = gridDim.x;
     const int nfull = (nwg / 8) * 8;
     if (bid < nfull) {
         bid = (bid % 8) * (nwg / 8) + bid / 8;
     }
-    const int gid = (bid * 4 + wave) * 8 + g;
-    const bool active = gid < A.ngroups;
-    const int gsafe = active ? gid : 0;
+    return bid;
+}
 
-    uint32_t ent[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        ent[i] = A.groups[(long long)gsafe * 8 + i];
-    }
-
+// Collaborative filtering of one group by its 8 lanes (lane c = block
+// column): gather, 3-D transform, hard threshold or Wiener shrinkage,
+// inverse (collaborative_hard / collaborative_wiener, src/bm3d.comp:384-443).
+// On return d[i*8 + j] is the filtered block i, row j, column c and the
+// result is the group's adaptive weight.
+template <int CM, bool FINAL>
+__device__ __forceinline__ float filter_group(const EstimateArgs &A, const uint32_t *ent, float *tb, int c,
+                                              float *d)
+{
     // gather: d[i*8 + j] = block i, row j, column c (src/bm3d.comp:677-689)
-    float d[64];
     float rf[FINAL ? 64 : 1];
     const long long pitch = A.pitch;
 #pragma unroll
@@ -627,7 +623,7 @@ __global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
         }
     }
 
-    // forward: vertical, horizontal, group axis (collaborative_hard / _wiener)
+    // forward: vertical, horizontal, group axis
     transform_rows<CM, false>(d);
     transpose_group(d, tb, c);
     transform_rows<CM, false>(d);
@@ -677,25 +673,204 @@ __global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
     transform_rows<CM, true>(d);
     transpose_group(d, tb, c);
     transform_blocks<CM, true>(d);
+    return w;
+}
 
-    // weighted scatter (src/bm3d.comp:697-715)
+// weighted scatter of one block of the group into a dense estimate slice
+// with device-scope atomics (src/bm3d.comp:697-715)
+template <int CM>
+__device__ __forceinline__ void scatter_global(const EstimateArgs &A, uint32_t e, const float *blk, float w,
+                                               float wq, int c)
+{
+    const long long rp = A.res_pitch;
+    const long long pe = (long long)A.height * rp;
+    float *wd = A.res + (long long)entry_z(e) * 2 * pe + (long long)entry_y(e) * rp + entry_x(e) + c;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float v = fma_c<CM>(w, blk[j], A.extractor) - A.extractor;
+        unsafeAtomicAdd(wd + j * rp, v);
+        unsafeAtomicAdd(wd + pe + j * rp, wq);
+    }
+}
+
+// Fallback for search ranges whose tile accumulator would not fit in LDS:
+// one group per 8 lanes in raster order, every contribution a global atomic.
+template <int CM, bool FINAL>
+__global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
+{
+    __shared__ float tile[4][8][72];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 3;  // group within the wavefront
+    const int c = lane & 7;   // column inside the block
+    float *tb = tile[wave][g];
+
+    const int bid = xcd_block_id();
+    const int gid = (bid * 4 + wave) * 8 + g;
+    const bool active = gid < A.ngroups;
+    const int gsafe = active ? gid : 0;
+
+    uint32_t ent[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        ent[i] = A.groups[(long long)gsafe * 8 + i];
+    }
+    float d[64];
+    const float w = filter_group<CM, FINAL>(A, ent, tb, c, d);
     if (active) {
-        const float E = A.extractor;
-        const float wq = (w + E) - E;
-        const long long rp = A.res_pitch;
-        const long long pe = (long long)A.height * rp;
+        const float wq = (w + A.extractor) - A.extractor;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            float *wd = A.res + (long long)entry_z(ent[i]) * 2 * pe + (long long)entry_y(ent[i]) * rp +
-                        entry_x(ent[i]) + c;
+            scatter_global<CM>(A, ent[i], d + i * 8, w, wq, c);
+        }
+    }
+}
+
+// Tiled estimate: a workgroup filters the groups of a tile of tile_bx x
+// tile_by reference blocks.  Their blocks in the reference block's own frame
+// lie within bm_range of it, so those contributions are summed in an LDS
+// accumulator covering the tile plus that margin, then written out
+// (non-atomically) to the tile's slot of the scratch buffer; the
+// bm3d_tile_reduce pass adds the overlapping tiles in a fixed order.  Blocks
+// in the other frames of the temporal window go to the dense slices with
+// global atomics.
+// Q31.32 fixed point of v, rounded toward zero: exact for every value the
+// extractor produces (multiples of 2^-22), and integer sums do not depend on
+// the order of the additions.
+__device__ __forceinline__ long long to_fixed(float v)
+{
+    const uint32_t b = __float_as_uint(v);
+    const int e = int((b >> 23) & 0xFFu);
+    const uint64_t m = uint64_t((b & 0x7FFFFFu) | 0x800000u);
+    const int sh = e - 118;  // v * 2^32 = m * 2^(e - 150 + 32)
+    uint64_t mag = sh >= 0 ? (m << min(sh, 39)) : (m >> min(-sh, 63));
+    mag = e == 0 ? 0 : mag;  // zero and subnormals
+    return (b >> 31) ? -(long long)mag : (long long)mag;
+}
+
+constexpr int kTiledThreads = 512;
+constexpr int kTiledWaves = kTiledThreads / 64;
+constexpr int kTiledGroupsPerPass = kTiledWaves * 8;
+
+// Tiled estimate: a workgroup filters the groups of a tile of tile_bx x
+// tile_by reference blocks.  Their blocks in the reference block's own frame
+// lie within bm_range of it, so those contributions are summed in an LDS
+// accumulator covering the tile plus that margin -- in 64-bit fixed point, as
+// float atomics in LDS are slow on this chip and integer sums are exact and
+// order-free -- then written out as floats (non-atomically) to the tile's
+// slot of the scratch buffer; the bm3d_tile_reduce pass adds the overlapping
+// tiles in a fixed order.  Blocks in the other frames of the temporal window
+// go to the dense slices with global atomics.
+template <int CM, bool FINAL>
+__global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArgs A)
+{
+    extern __shared__ __align__(16) unsigned long long acc[];  // [2][region_h][region_w]
+    __shared__ float tile[kTiledWaves][8][72];
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 3;
+    const int c = lane & 7;
+    float *tb = tile[wave][g];
+
+    const int bid = xcd_block_id();
+    const int tx = bid % A.tiles_x;
+    const int ty = bid / A.tiles_x;
+    int x_lo, x_hi, y_lo, y_hi;
+    tile_span(tx, A.tile_bx, A.nx, A.step, A.width, A.halo, &x_lo, &x_hi);
+    tile_span(ty, A.tile_by, A.ny, A.step, A.height, A.halo, &y_lo, &y_hi);
+    const int rs = A.region_w;
+    const int plane = A.region_h * rs;
+    for (int i = threadIdx.x; i < 2 * plane; i += kTiledThreads) {
+        acc[i] = 0ull;
+    }
+    __syncthreads();
+
+    const int bx0 = tx * A.tile_bx, by0 = ty * A.tile_by;
+    const int nbx = min(A.tile_bx, A.nx - bx0);
+    const int nby = min(A.tile_by, A.ny - by0);
+    const int nt = nbx * nby;
+    const int rw = x_hi - x_lo + 1, rh = y_hi - y_lo + 1;
+    for (int k0 = 0; k0 < nt; k0 += kTiledGroupsPerPass) {
+        const int k = k0 + wave * 8 + g;
+        const bool active = k < nt;
+        const int kk = active ? k : 0;
+        const int gid = (by0 + kk / nbx) * A.nx + bx0 + kk % nbx;
+        uint32_t ent[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const float v = fma_c<CM>(w, d[i * 8 + j], E) - E;
-                unsafeAtomicAdd(wd + j * rp, v);
-                unsafeAtomicAdd(wd + pe + j * rp, wq);
+        for (int i = 0; i < 8; ++i) {
+            ent[i] = A.groups[(long long)gid * 8 + i];
+        }
+        float d[64];
+        const float w = filter_group<CM, FINAL>(A, ent, tb, c, d);
+        if (active) {
+            const float E = A.extractor;
+            const float wq = (w + E) - E;
+            const unsigned long long wq_fixed = (unsigned long long)to_fixed(wq);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int lx = int(entry_x(ent[i])) - x_lo;
+                const int ly = int(entry_y(ent[i])) - y_lo;
+                // blocks of the frame itself always lie in the region, except
+                // the (0, 0) placeholders of windows with fewer than 8
+                // candidates; those take the global path
+                const bool local = int(entry_z(ent[i])) == A.radius && lx >= 0 && lx <= rw - 8 && ly >= 0 &&
+                                   ly <= rh - 8;
+                if (local) {
+                    unsigned long long *a = acc + ly * rs + lx + c;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float v = fma_c<CM>(w, d[i * 8 + j], E) - E;
+                        __hip_atomic_fetch_add(a + j * rs, (unsigned long long)to_fixed(v), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(a + plane + j * rs, wq_fixed, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                } else {
+                    scatter_global<CM>(A, ent[i], d + i * 8, w, wq, c);
+                }
             }
         }
     }
+    __syncthreads();
+    float *out = A.scratch + (long long)bid * 2 * plane;
+    const double scale = 1.0 / 4294967296.0;
+    for (int i = threadIdx.x; i < 2 * plane; i += kTiledThreads) {
+        out[i] = float(double((long long)acc[i]) * scale);
+    }
+}
+
+// Dense (wdst, weight) planes of the reference frame's own slice from the
+// tile accumulators: every pixel adds the tiles whose region covers it, in
+// tile order.
+__global__ __launch_bounds__(256) void bm3d_tile_reduce(ReduceArgs A)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= A.width || y >= A.height) {
+        return;
+    }
+    const int2 cr = A.col_tiles[x];
+    const int2 rr = A.row_tiles[y];
+    const long long plane = (long long)A.region_h * A.region_w;
+    float wd = 0.0f, wt = 0.0f;
+    for (int ty = rr.x; ty <= rr.y; ++ty) {
+        const int ly = y - A.tile_y0[ty];
+        for (int tx = cr.x; tx <= cr.y; ++tx) {
+            const int lx = x - A.tile_x0[tx];
+            const float *t = A.scratch + (long long)(ty * A.tiles_x + tx) * 2 * plane + (long long)ly * A.region_w + lx;
+            wd += t[0];
+            wt += t[plane];
+        }
+    }
+    const long long q = (long long)y * A.res_pitch + x;
+    const long long qw = q + (long long)A.height * A.res_pitch;
+    if (A.add_existing) {
+        // the slice also holds atomically added blocks (placeholder entries)
+        wd += A.res[q];
+        wt += A.res[qw];
+    }
+    A.res[q] = wd;
+    A.res[qw] = wt;
 }
 
 // dst = float(sum double(wdst) / sum double(weight)) over the tw slices
@@ -753,12 +928,20 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
 
 hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, hipStream_t s)
 {
-    const dim3 grid((a.ngroups + 31) / 32), block(256);
-#define VSFEEL_EST(CM)                                                             \
-    if (final_pass) {                                                              \
-        hipLaunchKernelGGL((bm3d_estimate<CM, true>), grid, block, 0, s, a);       \
-    } else {                                                                       \
-        hipLaunchKernelGGL((bm3d_estimate<CM, false>), grid, block, 0, s, a);      \
+    const bool tiled = a.scratch != nullptr;
+    const dim3 grid(tiled ? a.tiles_x * a.tiles_y : (a.ngroups + 31) / 32), block(tiled ? kTiledThreads : 256);
+    const size_t lds = tiled ? size_t(2) * a.region_h * a.region_w * sizeof(unsigned long long) : 0;
+#define VSFEEL_EST(CM)                                                                         \
+    if (tiled) {                                                                               \
+        if (final_pass) {                                                                      \
+            hipLaunchKernelGGL((bm3d_estimate_tiled<CM, true>), grid, block, lds, s, a);       \
+        } else {                                                                               \
+            hipLaunchKernelGGL((bm3d_estimate_tiled<CM, false>), grid, block, lds, s, a);      \
+        }                                                                                      \
+    } else if (final_pass) {                                                                   \
+        hipLaunchKernelGGL((bm3d_estimate<CM, true>), grid, block, 0, s, a);                   \
+    } else {                                                                                   \
+        hipLaunchKernelGGL((bm3d_estimate<CM, false>), grid, block, 0, s, a);                  \
     }
     if (fp_mode == kFpNone) {
         VSFEEL_EST(kFpNone)
@@ -768,6 +951,13 @@ hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, 
         VSFEEL_EST(kFpExplicit)
     }
 #undef VSFEEL_EST
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s)
+{
+    const dim3 grid((a.width + 63) / 64, (a.height + 3) / 4), block(256);
+    hipLaunchKernelGGL(bm3d_tile_reduce, grid, block, 0, s, a);
     return hipGetLastError();
 }
 

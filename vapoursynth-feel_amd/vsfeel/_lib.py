@@ -31,7 +31,7 @@ EXPORTS = (
     "vsfeel_bm3d_kernel_times",
 )
 
-KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate")
+KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce")
 
 
 class Format(ctypes.Structure):

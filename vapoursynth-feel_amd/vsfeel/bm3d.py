@@ -116,9 +116,10 @@ class _Bm3dClip(Clip):
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h:
-            _lib.load().vsfeel_bm3d_free(h)
-            self._handle = None
+        lib = _lib._lib
+        if h and lib is not None:
+            lib.vsfeel_bm3d_free(h)
+        self._handle = None
 
     def get_frame(self, n: int) -> list[np.ndarray]:
         L = _lib.load()

@@ -64,9 +64,10 @@ class Instance:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h:
-            _lib.load().vsfeel_bm3d_free(h)
-            self.handle = None
+        lib = _lib._lib
+        if h and lib is not None:
+            lib.vsfeel_bm3d_free(h)
+        self.handle = None
 
     def _clip(self, x: torch.Tensor, ref: torch.Tensor | None) -> _lib.DeviceClip:
         _check(x, self)
@@ -106,8 +107,9 @@ class Instance:
 
     def kernel_times(self) -> dict:
         """{kernel: (total ms, launches)} of the launches recorded so far."""
-        ms = (ctypes.c_double * 4)()
-        cnt = (ctypes.c_int64 * 4)()
+        n = len(_lib.KERNEL_NAMES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
         rc = _lib.load().vsfeel_bm3d_kernel_times(self.handle, ms, cnt)
         if rc != _lib.OK:
             raise Error("BM3D: kernel timing failed")
