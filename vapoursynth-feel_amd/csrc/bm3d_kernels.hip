@@ -66,6 +66,19 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l)
     return uint32_t(__builtin_amdgcn_readlane(int(v), l));
 }
 
+// DPP lane moves (full-rate VALU, no LDS round trip).
+// row_shr:1 -- lane i takes lane i-1 of its 16-lane row (row's lane 0 keeps its own)
+__device__ __forceinline__ uint32_t dpp_row_shr1(uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x111, 0xF, 0xF, false));
+}
+
+// wave_shl:1 -- lane i takes lane i+1 across the whole wavefront (lane 63 keeps its own)
+__device__ __forceinline__ float dpp_wave_shl1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l)
 {
     return (uint64_t(readlane_u32(uint32_t(v >> 32), l)) << 32) | readlane_u32(uint32_t(v), l);
@@ -95,8 +108,8 @@ __device__ __forceinline__ void list_insert(TopList &L, uint64_t key, uint32_t x
 {
     const uint64_t below = __ballot(lane < 8 && L.key < key);
     const int pos = __popcll(below & 0xFFull);
-    const uint64_t up_key = shfl_up64(L.key, 1);
-    const uint32_t up_xy = uint32_t(__shfl_up(int(L.xy), 1));
+    const uint64_t up_key = (uint64_t(dpp_row_shr1(uint32_t(L.key >> 32))) << 32) | dpp_row_shr1(uint32_t(L.key));
+    const uint32_t up_xy = dpp_row_shr1(L.xy);
     if (lane == pos) {
         L.key = key;
         L.xy = xy;
@@ -210,7 +223,7 @@ __device__ __forceinline__ float col_ssd(const float *c, const float *s)
 template <int CM, bool EXACT>
 __device__ void scan_window(TopList &L, const float *__restrict__ plane, int pitch, int height,
                             const float *cur_lds, const float *cur_reg, int left, int right,
-                            int top, int bottom, int seq_base, int lane)
+                            int top, int bottom, int seq_base, int lane, int skip_row = -1)
 {
     const int rw = right - left + 1;
     const int rh = bottom - top + 1;
@@ -253,9 +266,9 @@ __device__ void scan_window(TopList &L, const float *__restrict__ plane, int pit
             float d;
             if (!EXACT) {
                 const float col = col_ssd<CM>(c, s);
-                const float p1 = col + __shfl_down(col, 1);
-                const float p2 = p1 + __shfl_down(p1, 2);
-                d = p2 + __shfl_down(p2, 4);
+                const float p1 = col + dpp_wave_shl1(col);
+                const float p2 = p1 + dpp_wave_shl1(dpp_wave_shl1(p1));
+                d = p2 + dpp_wave_shl1(dpp_wave_shl1(dpp_wave_shl1(dpp_wave_shl1(p2))));
             } else {
                 float S[8];
 #pragma unroll
@@ -269,7 +282,7 @@ __device__ void scan_window(TopList &L, const float *__restrict__ plane, int pit
                 d = (t01 + t23) + (t45 + t67);
             }
             const int row = cy - top;
-            const bool cand = lane_on && kk < nc && row < rh;
+            const bool cand = lane_on && kk < nc && row < rh && cy != skip_row;
             const int seq = seq_base + row * rw + j;
             const uint32_t xy = uint32_t(left + j) | (uint32_t(cy) << 16);
             list_offer(L, cand, d, seq, xy, lane);
@@ -353,7 +366,13 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
         const int R = A.bm_range;
         const int left = max(x - R, 0), right = min(x + R, W - 8);
         const int top = max(y - R, 0), bottom = min(y + R, H - 8);
-        scan_window<CM, EXACT>(L, centre, pitch, H, cur_lds, cur_reg, left, right, top, bottom, 0, lane);
+        // the reference block's own row first: its candidates are usually
+        // among the best, so the top-8 threshold is tight from the start and
+        // few later candidates survive it (the result does not depend on the
+        // order: the list keeps the 8 smallest (distance, seq) keys)
+        const int rw = right - left + 1;
+        scan_window<CM, EXACT>(L, centre, pitch, H, cur_lds, cur_reg, left, right, y, y, (y - top) * rw, lane);
+        scan_window<CM, EXACT>(L, centre, pitch, H, cur_lds, cur_reg, left, right, top, bottom, 0, lane, y);
     }
     GroupList G;
     G.e = __uint_as_float(uint32_t(L.key >> 32));
