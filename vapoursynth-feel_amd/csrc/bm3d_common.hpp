@@ -28,6 +28,13 @@ struct MatchArgs {
     int width, height, pitch;
     int step, bm_range, radius, ps_num, ps_range;
     int nx, ny;                      // active ref-block grid
+    // spatial top-8 per ref block (distance bits, x | y << 16): written by
+    // bm3d_match_rows when the temporal search follows, read by bm3d_match
+    uint2 *spatial;
+    // bm3d_match_rows geometry: ref blocks per workgroup run, and its LDS
+    // layout (column-major source region, per-block distance rows)
+    int run_blocks, runs_x;
+    int region_cols, region_rows, dist_cols;
 };
 
 // Collaborative filtering + weighted scatter of every group.

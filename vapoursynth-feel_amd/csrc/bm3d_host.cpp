@@ -113,6 +113,12 @@ struct Instance {
     float *res_buf = nullptr;    // res_cap stacks of tw * 2 planes
     uint32_t *groups = nullptr;  // nref * 8
     float *dists = nullptr;      // nref * 8 (debug)
+    uint2 *spatial = nullptr;    // nref * 8 spatial top-8 handed to the temporal search
+
+    // row-parallel spatial search (bm3d_match_rows): ref blocks per
+    // workgroup run and the LDS layout; run_blocks == 0 selects the
+    // per-wavefront search instead
+    int run_blocks = 0, runs_x = 0, region_cols = 0, region_rows = 0, dist_cols = 0;
     float *dst_dev = nullptr;    // one plane
 
     // tiled accumulation of the frame's own slice (see bm3d_estimate_tiled)
@@ -217,6 +223,7 @@ struct Instance {
         (void)hipFree(res_buf);
         (void)hipFree(groups);
         (void)hipFree(dists);
+        (void)hipFree(spatial);
         (void)hipFree(dst_dev);
         (void)hipFree(scratch);
         (void)hipFree(tile_tables);
@@ -239,20 +246,8 @@ struct Instance {
     // one estimation dispatch (record_bm3d_kernels, src/bm3d.cpp:558-602):
     // clear the stack, block matching, collaborative filtering
     // ------------------------------------------------------------------
-    hipError_t estimate(const float *const *search, const float *const *input, int in_pitch,
-                        float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr)
+    MatchArgs match_args(const float *const *search, int in_pitch, float *dists_out) const
     {
-        hipError_t e = hipSuccess;
-        if (!tiled || radius > 0 || short_windows) {
-            // the tile reduction overwrites the frame's own slice; the other
-            // slices are accumulated with atomics and must start at zero
-            e = timed_launch(kKernelFill, s, [&] {
-                return launch_fill_zero(stack, (long long)tw * 2 * height * stack_pitch, s);
-            });
-            if (e != hipSuccess) {
-                return e;
-            }
-        }
         MatchArgs m {};
         for (int z = 0; z < tw; ++z) {
             m.plane[z] = search[z];
@@ -269,6 +264,30 @@ struct Instance {
         m.ps_range = ps_range;
         m.nx = nx;
         m.ny = ny;
+        m.spatial = spatial;
+        m.run_blocks = run_blocks;
+        m.runs_x = runs_x;
+        m.region_cols = region_cols;
+        m.region_rows = region_rows;
+        m.dist_cols = dist_cols;
+        return m;
+    }
+
+    hipError_t estimate(const float *const *search, const float *const *input, int in_pitch,
+                        float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr)
+    {
+        hipError_t e = hipSuccess;
+        if (!tiled || radius > 0 || short_windows) {
+            // the tile reduction overwrites the frame's own slice; the other
+            // slices are accumulated with atomics and must start at zero
+            e = timed_launch(kKernelFill, s, [&] {
+                return launch_fill_zero(stack, (long long)tw * 2 * height * stack_pitch, s);
+            });
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
+        const MatchArgs m = match_args(search, in_pitch, dists_out);
         e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
         if (e != hipSuccess) {
             return e;
@@ -523,6 +542,24 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
         const int rh = std::min(d.bm_range, d.height - 8) + 1;
         d.short_windows = (long long)rw * rh < 8;
     }
+    if (d.radius < 3) {
+        // one window row per thread of a 256-thread workgroup; the source
+        // region, the blocks and their distance rows must fit 64 KB of LDS
+        const int rh = std::min(2 * d.bm_range + 1, d.height - 7);
+        const int nb = rh <= 256 ? 256 / rh : 0;
+        if (nb > 0) {
+            const int cols = std::min((nb - 1) * d.block_step + 8 + 2 * d.bm_range, d.width);
+            const int dcols = std::min(2 * d.bm_range + 1, d.width - 7);
+            const long long bytes = 4LL * ((long long)cols * (rh + 7) + nb * 64LL + (long long)nb * rh * dcols);
+            if (bytes <= 64 * 1024) {
+                d.run_blocks = nb;
+                d.runs_x = (d.nx + nb - 1) / nb;
+                d.region_cols = cols;
+                d.region_rows = rh + 7;
+                d.dist_cols = dcols;
+            }
+        }
+    }
     d.tiled = choose_tiles(d.width, d.height, d.block_step, d.bm_range, &d.tile_bx, &d.tile_by, &d.region_w,
                            &d.region_h);
     if (d.tiled) {
@@ -588,6 +625,7 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         (e = hipMalloc(&d.res_buf, plane * d.res_cap * d.tw * 2)) != hipSuccess ||
         (e = hipMalloc(&d.groups, size_t(d.nref) * 8 * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&d.dists, size_t(d.nref) * 8 * sizeof(float))) != hipSuccess ||
+        (d.radius > 0 && (e = hipMalloc(&d.spatial, size_t(d.nref) * 8 * sizeof(uint2))) != hipSuccess) ||
         (e = hipMalloc(&d.dst_dev, plane)) != hipSuccess) {
         return dev_fail(e, "hipMalloc");
     }
@@ -857,22 +895,7 @@ int vsfeel_bm3d_debug_groups(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip
     const float *search[kMaxWindow];
     const float *input[kMaxWindow];
     device_window(d, clip, m, search, input);
-    MatchArgs a {};
-    for (int z = 0; z < d.tw; ++z) {
-        a.plane[z] = search[z];
-    }
-    a.groups = d.groups;
-    a.dists = d.dists;
-    a.width = d.width;
-    a.height = d.height;
-    a.pitch = clip->pitch;
-    a.step = d.block_step;
-    a.bm_range = d.bm_range;
-    a.radius = d.radius;
-    a.ps_num = d.ps_num;
-    a.ps_range = d.ps_range;
-    a.nx = d.nx;
-    a.ny = d.ny;
+    const MatchArgs a = d.match_args(search, clip->pitch, d.dists);
     VSFEEL_HIP(launch_match(a, d.fp_mode, s), "match launch");
     std::vector<uint32_t> packed(size_t(d.nref) * 8);
     VSFEEL_HIP(hipMemcpyAsync(packed.data(), d.groups, packed.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpy");

@@ -26,6 +26,20 @@ constexpr int kSortThreshold = 8;  // survivors above this are merged by a sort
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// XCD-aware workgroup order: the hardware hands consecutive workgroups to
+// the 8 XCDs in turn, so give every XCD a contiguous run of the work (its L2
+// then holds one band of the frame)
+__device__ __forceinline__ int xcd_block_id()
+{
+    int bid = blockIdx.x;
+    const int nwg = gridDim.x;
+    const int nfull = (nwg / 8) * 8;
+    if (bid < nfull) {
+        bid = (bid % 8) * (nwg / 8) + bid / 8;
+    }
+    return bid;
+}
+
 // Orders LDS traffic of one wavefront: LDS operations of a wave execute in
 // issue order, so only the compiler has to be kept from moving them across.
 __device__ __forceinline__ void wave_lds_sync()
@@ -320,6 +334,217 @@ __device__ __forceinline__ void group_insert(GroupList &G, float fe, uint32_t fx
     }
 }
 
+// self-block rule (src/bm3d.comp:639-663) and the group table write
+__device__ __forceinline__ void finish_group(const MatchArgs &A, GroupList &G, int blk, int x, int y, int lane)
+{
+    const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
+    const uint64_t present = __ballot(lane < 8 && G.xy == self_xy && G.z == A.radius);
+    if (present == 0) {
+        const float up_e = __shfl_up(G.e, 1);
+        const uint32_t up_xy = uint32_t(__shfl_up(int(G.xy), 1));
+        const int up_z = __shfl_up(G.z, 1);
+        if (lane == 0) {
+            G.e = -1.0f;
+            G.xy = self_xy;
+            G.z = A.radius;
+        } else if (lane < 8) {
+            G.e = up_e;
+            G.xy = up_xy;
+            G.z = up_z;
+        }
+    }
+    if (lane < 8) {
+        const uint32_t gx = G.xy & 0xFFFFu, gy = G.xy >> 16;
+        A.groups[(long long)blk * 8 + lane] = pack_entry(gx, gy, uint32_t(G.z));
+        if (A.dists) {
+            A.dists[(long long)blk * 8 + lane] = G.e;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spatial search of the sliding metric, row-parallel.  A workgroup takes a
+// run of run_blocks consecutive ref blocks of one grid row; their windows
+// share the rows [top, bottom], so one source region (column-major in LDS)
+// serves them all.  Phase 1: every thread owns one (block, window row) and
+// walks the row: column SSDs v_j against reference column min(j, 7), then
+// D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
+// written to the block's distance rows in LDS.  Phase 2: one wavefront per
+// block picks the top-8 (distance, seq) keys: the 8th smallest of the 64
+// lane minima bounds the 8th smallest distance, and only candidates at or
+// below it are offered to the list.
+// ---------------------------------------------------------------------------
+template <int CM>
+__global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
+{
+    extern __shared__ __align__(16) float lds[];
+    const int NB = A.run_blocks;
+    const int CS = A.region_rows;  // column stride of the region
+    float *region = lds;                                  // [region_cols][region_rows]
+    float *cur = region + A.region_cols * CS;             // [NB][64]
+    float *dist = cur + NB * 64;                          // [NB][rh][dist_cols]
+
+    const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
+    const int run = xcd_block_id();
+    const int by = run / A.runs_x;
+    const int bx0 = (run - by * A.runs_x) * NB;
+    const int nb = min(NB, A.nx - bx0);
+    const int y = min(by * s, H - 8);
+    const int top = max(y - R, 0), bottom = min(y + R, H - 8);
+    const int rh = bottom - top + 1;
+    const int x_first = min(bx0 * s, W - 8);
+    const int x_last = min((bx0 + nb - 1) * s, W - 8);
+    const int rx0 = max(x_first - R, 0);
+    const int rx1 = min(x_last + R, W - 8) + 7;
+    const float *plane = A.plane[A.radius];
+
+    // stage the source region (rows top .. bottom+7, columns rx0 .. rx1)
+    const int ncols = rx1 - rx0 + 1, nrows = rh + 7;
+    for (int i = threadIdx.x; i < ncols * nrows; i += 256) {
+        const int row = i / ncols, col = i - row * ncols;
+        region[col * CS + row] = plane[(long long)(top + row) * pitch + rx0 + col];
+    }
+    for (int i = threadIdx.x; i < nb * 64; i += 256) {
+        const int b = i >> 6, e = i & 63;
+        const int xb = min((bx0 + b) * s, W - 8);
+        cur[i] = plane[(long long)(y + (e & 7)) * pitch + xb + (e >> 3)];  // cur[c*8+i]
+    }
+    __syncthreads();
+
+    // phase 1: one window row per thread
+    const int t = threadIdx.x;
+    if (t < nb * rh) {
+        const int b = t / rh, r = t - b * rh;
+        const int xb = min((bx0 + b) * s, W - 8);
+        const int left = max(xb - R, 0), right = min(xb + R, W - 8);
+        const int rw = right - left + 1;
+        const float *cb = cur + b * 64;
+        float c7[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            c7[i] = cb[56 + i];
+        }
+        const float *col0 = region + (left - rx0) * CS + r;  // column j at col0 + j*CS
+        float *drow = dist + (b * rh + r) * A.dist_cols;
+        float v[15];
+        for (int k0 = 0; k0 < rw; k0 += 8) {
+            if (k0 == 0) {
+#pragma unroll
+                for (int j = 0; j < 15; ++j) {
+                    if (j <= rw + 6) {
+                        const float *sp = col0 + j * CS;
+                        float sv[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            sv[i] = sp[i];
+                        }
+                        if (j < 7) {
+                            float cc[8];
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                cc[i] = cb[j * 8 + i];
+                            }
+                            v[j] = col_ssd<CM>(cc, sv);
+                        } else {
+                            v[j] = col_ssd<CM>(c7, sv);
+                        }
+                    } else {
+                        v[j] = 0.0f;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    v[j] = v[j + 8];
+                }
+#pragma unroll
+                for (int j = 7; j < 15; ++j) {
+                    if (k0 + j <= rw + 6) {
+                        const float *sp = col0 + (k0 + j) * CS;
+                        float sv[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            sv[i] = sp[i];
+                        }
+                        v[j] = col_ssd<CM>(c7, sv);
+                    } else {
+                        v[j] = 0.0f;
+                    }
+                }
+            }
+            float p1[14], p2[12];
+#pragma unroll
+            for (int j = 0; j < 14; ++j) {
+                p1[j] = v[j] + v[j + 1];
+            }
+#pragma unroll
+            for (int j = 0; j < 12; ++j) {
+                p2[j] = p1[j] + p1[j + 2];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (k0 + i < rw) {
+                    drow[k0 + i] = p2[i] + p2[i + 4];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase 2: top-8 of each block by one wavefront
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    for (int b = wave; b < nb; b += 4) {
+        const int bx = bx0 + b;
+        const int xb = min(bx * s, W - 8);
+        const int left = max(xb - R, 0), right = min(xb + R, W - 8);
+        const int rw = right - left + 1;
+        const int n = rw * rh;
+        const float *db = dist + b * rh * A.dist_cols;
+        // lane minimum over a strided share of the candidates
+        float lmin = FLT_MAX;
+        for (int q = lane; q < n; q += 64) {
+            const int r = q / rw, k = q - r * rw;
+            const float d = db[r * A.dist_cols + k];
+            lmin = (d < lmin) ? d : lmin;  // FLT_MAX itself never counts (strict '<')
+        }
+        // 8th smallest lane minimum: a valid upper bound of the 8th distance
+        float key = lmin;
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const float o = __shfl_xor(key, j);
+                const bool asc = (lane & k) == 0;
+                const bool lower = (lane & j) == 0;
+                key = (lower == asc) ? fminf(key, o) : fmaxf(key, o);
+            }
+        }
+        const float thr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), 7));
+        TopList L;
+        list_reset(L);
+        for (int q0 = 0; q0 < n; q0 += 64) {
+            const int q = q0 + lane;
+            const int r = q / rw, k = q - r * rw;
+            const bool in = q < n;
+            const float d = in ? db[r * A.dist_cols + k] : FLT_MAX;
+            const uint32_t xy = uint32_t(left + k) | (uint32_t(top + r) << 16);
+            list_offer(L, in && d <= thr, d, q, xy, lane);
+        }
+        if (A.spatial) {
+            if (lane < 8) {
+                A.spatial[(long long)(by * A.nx + bx) * 8 + lane] = make_uint2(uint32_t(L.key >> 32), L.xy);
+            }
+        } else {
+            GroupList G;
+            G.e = __uint_as_float(uint32_t(L.key >> 32));
+            G.xy = L.xy;
+            G.z = A.radius;
+            finish_group(A, G, by * A.nx + bx, xb, y, lane);
+        }
+    }
+}
+
 template <int CM, bool EXACT>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
 {
@@ -327,15 +552,7 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
 
-    // XCD-aware order: consecutive workgroups go to different XCDs, so give
-    // every XCD a contiguous band of the frame (its L2 then holds the band)
-    int bid = blockIdx.x;
-    const int nwg = gridDim.x;
-    const int nfull = (nwg / 8) * 8;
-    if (bid < nfull) {
-        bid = (bid % 8) * (nwg / 8) + bid / 8;
-    }
-    const int blk = bid * 4 + wave;
+    const int blk = xcd_block_id() * 4 + wave;
     const int nref = A.nx * A.ny;
     if (blk >= nref) {
         return;  // whole wavefront; no block-wide barrier follows
@@ -362,7 +579,14 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
     // spatial search (src/bm3d.comp:479-538)
     TopList L;
     list_reset(L);
-    {
+    if (A.spatial) {
+        // done by bm3d_match_rows
+        const uint2 e = A.spatial[(long long)blk * 8 + (lane & 7)];
+        if (lane < 8) {
+            L.key = uint64_t(e.x) << 32;
+            L.xy = e.y;
+        }
+    } else {
         const int R = A.bm_range;
         const int left = max(x - R, 0), right = min(x + R, W - 8);
         const int top = max(y - R, 0), bottom = min(y + R, H - 8);
@@ -409,30 +633,7 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
         }
     }
 
-    // the reference block itself is always part of the group (src/bm3d.comp:639-663)
-    const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
-    const uint64_t present = __ballot(lane < 8 && G.xy == self_xy && G.z == r);
-    if (present == 0) {
-        const float up_e = __shfl_up(G.e, 1);
-        const uint32_t up_xy = uint32_t(__shfl_up(int(G.xy), 1));
-        const int up_z = __shfl_up(G.z, 1);
-        if (lane == 0) {
-            G.e = -1.0f;
-            G.xy = self_xy;
-            G.z = r;
-        } else if (lane < 8) {
-            G.e = up_e;
-            G.xy = up_xy;
-            G.z = up_z;
-        }
-    }
-    if (lane < 8) {
-        const uint32_t gx = G.xy & 0xFFFFu, gy = G.xy >> 16;
-        A.groups[(long long)blk * 8 + lane] = pack_entry(gx, gy, uint32_t(G.z));
-        if (A.dists) {
-            A.dists[(long long)blk * 8 + lane] = G.e;
-        }
-    }
+    finish_group(A, G, blk, x, y, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -600,20 +801,6 @@ __device__ __forceinline__ float group8_sum_tree(float k)
     k = k + __shfl_xor(k, 2);
     k = k + __shfl_xor(k, 4);
     return k;
-}
-
-// XCD-aware workgroup order: the hardware hands consecutive workgroups to
-// the 8 XCDs in turn, so give every XCD a contiguous run of the work (its L2
-// then holds one band of the frame)
-__device__ __forceinline__ int xcd_block_id()
-{
-    int bid = blockIdx.x;
-    const int nwg = gridDim.x;
-    const int nfull = (nwg / 8) * 8;
-    if (bid < nfull) {
-        bid = (bid % 8) * (nwg / 8) + bid / 8;
-    }
-    return bid;
 }
 
 // Collaborative filtering of one group by its 8 lanes (lane c = block
@@ -929,17 +1116,45 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
     const dim3 grid((nref + 3) / 4), block(256);
     const bool exact = a.radius >= 3;
     const bool fused = fp_mode == kFpAll;
-    if (exact) {
-        if (fused) {
-            hipLaunchKernelGGL((bm3d_match<kFpAll, true>), grid, block, 0, s, a);
-        } else {
-            hipLaunchKernelGGL((bm3d_match<kFpExplicit, true>), grid, block, 0, s, a);
+    if (!exact && a.run_blocks > 0) {
+        // row-parallel spatial search; at radius 0 it writes the groups
+        // itself, otherwise it hands the spatial top-8 to the temporal search
+        MatchArgs rows = a;
+        if (a.radius == 0) {
+            rows.spatial = nullptr;
         }
-    } else {
+        const int rh = min(2 * a.bm_range + 1, a.height - 7);
+        const size_t lds = sizeof(float) * (size_t(a.region_cols) * a.region_rows + size_t(a.run_blocks) * 64 +
+                                            size_t(a.run_blocks) * rh * a.dist_cols);
+        const dim3 rgrid(a.ny * a.runs_x);
+        if (fused) {
+            hipLaunchKernelGGL((bm3d_match_rows<kFpAll>), rgrid, block, lds, s, rows);
+        } else {
+            hipLaunchKernelGGL((bm3d_match_rows<kFpExplicit>), rgrid, block, lds, s, rows);
+        }
+        if (a.radius == 0) {
+            return hipGetLastError();
+        }
         if (fused) {
             hipLaunchKernelGGL((bm3d_match<kFpAll, false>), grid, block, 0, s, a);
         } else {
             hipLaunchKernelGGL((bm3d_match<kFpExplicit, false>), grid, block, 0, s, a);
+        }
+        return hipGetLastError();
+    }
+    MatchArgs b = a;
+    b.spatial = nullptr;
+    if (exact) {
+        if (fused) {
+            hipLaunchKernelGGL((bm3d_match<kFpAll, true>), grid, block, 0, s, b);
+        } else {
+            hipLaunchKernelGGL((bm3d_match<kFpExplicit, true>), grid, block, 0, s, b);
+        }
+    } else {
+        if (fused) {
+            hipLaunchKernelGGL((bm3d_match<kFpAll, false>), grid, block, 0, s, b);
+        } else {
+            hipLaunchKernelGGL((bm3d_match<kFpExplicit, false>), grid, block, 0, s, b);
         }
     }
     return hipGetLastError();
