@@ -37,6 +37,17 @@ struct MatchArgs {
     int region_cols, region_rows, dist_cols;
 };
 
+// LDS bytes of bm3d_match_rows: survivor stages of its 4 wavefronts (64 keys
+// + positions each), the source region, the blocks, their distance rows and
+// row minima (region_rows - 7 window rows at most).
+__host__ __device__ inline long long rows_lds_bytes(int region_cols, int region_rows, int run_blocks, int dist_cols)
+{
+    const long long rh = region_rows - 7;
+    return 256LL * 8 + 256LL * 4 +
+           4LL * ((long long)region_cols * region_rows + run_blocks * 64LL + run_blocks * rh * dist_cols +
+                  run_blocks * rh);
+}
+
 // Collaborative filtering + weighted scatter of every group.
 struct EstimateArgs {
     const float *search[kMaxWindow]; // block-matching / Wiener reference clip

@@ -550,7 +550,7 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
         if (nb > 0) {
             const int cols = std::min((nb - 1) * d.block_step + 8 + 2 * d.bm_range, d.width);
             const int dcols = std::min(2 * d.bm_range + 1, d.width - 7);
-            const long long bytes = 4LL * ((long long)cols * (rh + 7) + nb * 64LL + (long long)nb * rh * dcols);
+            const long long bytes = rows_lds_bytes(cols, rh + 7, nb, dcols);
             if (bytes <= 64 * 1024) {
                 d.run_blocks = nb;
                 d.runs_x = (d.nx + nb - 1) / nb;

@@ -362,6 +362,96 @@ __device__ __forceinline__ void finish_group(const MatchArgs &A, GroupList &G, i
     }
 }
 
+// DPP move of v; lanes without a source lane in their row keep `old`
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(v), CTRL, 0xF, 0xF, false));
+}
+
+// minimum over the wavefront: DPP row shifts leave each 16-lane row's
+// minimum in its lane 15, then the four rows are combined
+__device__ __forceinline__ float wave_min(float v)
+{
+    const uint32_t inf = __float_as_uint(INFINITY);
+    v = fminf(v, __uint_as_float(dpp_u32<0x111>(inf, __float_as_uint(v))));  // row_shr:1
+    v = fminf(v, __uint_as_float(dpp_u32<0x112>(inf, __float_as_uint(v))));  // row_shr:2
+    v = fminf(v, __uint_as_float(dpp_u32<0x114>(inf, __float_as_uint(v))));  // row_shr:4
+    v = fminf(v, __uint_as_float(dpp_u32<0x118>(inf, __float_as_uint(v))));  // row_shr:8
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return fminf(fminf(r0, r1), fminf(r2, r3));
+}
+
+// 8th smallest of the 64 lane values, ties counted; FLT_MAX when fewer than
+// 8 lanes hold a value below FLT_MAX
+__device__ __forceinline__ float kth_smallest8(float a, int lane)
+{
+    (void)lane;
+    int taken = 0;
+    float m = FLT_MAX;
+    for (int it = 0; it < 8 && taken < 8; ++it) {
+        m = wave_min(a);
+        if (!(m < FLT_MAX)) {
+            return FLT_MAX;
+        }
+        const bool hit = a == m;
+        taken += __popcll(__ballot(hit));
+        a = hit ? INFINITY : a;
+    }
+    return m;
+}
+
+// lane value of the partner lane ^ J inside a 16-lane row
+template <int J>
+__device__ __forceinline__ uint32_t dpp_xor(uint32_t v, int lane)
+{
+    if (J == 1) {
+        return dpp_u32<0xB1>(v, v);  // quad_perm [1,0,3,2]
+    } else if (J == 2) {
+        return dpp_u32<0x4E>(v, v);  // quad_perm [2,3,0,1]
+    } else if (J == 4) {
+        const uint32_t up = dpp_u32<0x104>(v, v);    // row_shl:4 (lane i reads i+4)
+        const uint32_t down = dpp_u32<0x114>(v, v);  // row_shr:4 (lane i reads i-4)
+        return (lane & 4) ? down : up;
+    } else {
+        return dpp_u32<0x128>(v, v);  // row_ror:8 (lane i reads i^8)
+    }
+}
+
+template <int K, int J>
+__device__ __forceinline__ void sort16_step(uint64_t &key, uint32_t &xy, int lane)
+{
+    const uint32_t ohi = dpp_xor<J>(uint32_t(key >> 32), lane);
+    const uint32_t olo = dpp_xor<J>(uint32_t(key), lane);
+    const uint32_t oxy = dpp_xor<J>(xy, lane);
+    const uint64_t ok = (uint64_t(ohi) << 32) | olo;
+    const bool asc = ((lane & 15) & K) == 0;
+    const bool lower = (lane & J) == 0;
+    const bool take = (lower == asc) ? (ok < key) : (ok > key);
+    if (take) {
+        key = ok;
+        xy = oxy;
+    }
+}
+
+// bitonic sort, ascending, of the 16 keys of each 16-lane row
+__device__ __forceinline__ void sort16_dpp(uint64_t &key, uint32_t &xy, int lane)
+{
+    sort16_step<2, 1>(key, xy, lane);
+    sort16_step<4, 2>(key, xy, lane);
+    sort16_step<4, 1>(key, xy, lane);
+    sort16_step<8, 4>(key, xy, lane);
+    sort16_step<8, 2>(key, xy, lane);
+    sort16_step<8, 1>(key, xy, lane);
+    sort16_step<16, 8>(key, xy, lane);
+    sort16_step<16, 4>(key, xy, lane);
+    sort16_step<16, 2>(key, xy, lane);
+    sort16_step<16, 1>(key, xy, lane);
+}
+
 // ---------------------------------------------------------------------------
 // Spatial search of the sliding metric, row-parallel.  A workgroup takes a
 // run of run_blocks consecutive ref blocks of one grid row; their windows
@@ -377,12 +467,16 @@ __device__ __forceinline__ void finish_group(const MatchArgs &A, GroupList &G, i
 template <int CM>
 __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 {
-    extern __shared__ __align__(16) float lds[];
+    extern __shared__ __align__(16) unsigned long long lds64[];
     const int NB = A.run_blocks;
     const int CS = A.region_rows;  // column stride of the region
-    float *region = lds;                                  // [region_cols][region_rows]
-    float *cur = region + A.region_cols * CS;             // [NB][64]
-    float *dist = cur + NB * 64;                          // [NB][rh][dist_cols]
+    const int RH = A.region_rows - 7;
+    unsigned long long *stage_key = lds64;                                  // [4][64] survivors
+    uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);         // [4][64]
+    float *region = reinterpret_cast<float *>(stage_xy + 256);              // [region_cols][region_rows]
+    float *cur = region + A.region_cols * CS;                               // [NB][64]
+    float *dist = cur + NB * 64;                                            // [NB][rh][dist_cols]
+    float *rowmin = dist + NB * RH * A.dist_cols;                           // [NB][rh]
 
     const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
     const int run = xcd_block_id();
@@ -426,6 +520,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         }
         const float *col0 = region + (left - rx0) * CS + r;  // column j at col0 + j*CS
         float *drow = dist + (b * rh + r) * A.dist_cols;
+        float rmin = FLT_MAX;
         float v[15];
         for (int k0 = 0; k0 < rw; k0 += 8) {
             if (k0 == 0) {
@@ -484,10 +579,13 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 if (k0 + i < rw) {
-                    drow[k0 + i] = p2[i] + p2[i + 4];
+                    const float d = p2[i] + p2[i + 4];
+                    drow[k0 + i] = d;
+                    rmin = (d < rmin) ? d : rmin;  // FLT_MAX and NaN never count
                 }
             }
         }
+        rowmin[b * rh + r] = rmin;
     }
     __syncthreads();
 
@@ -501,36 +599,71 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int rw = right - left + 1;
         const int n = rw * rh;
         const float *db = dist + b * rh * A.dist_cols;
-        // lane minimum over a strided share of the candidates
-        float lmin = FLT_MAX;
-        for (int q = lane; q < n; q += 64) {
-            const int r = q / rw, k = q - r * rw;
-            const float d = db[r * A.dist_cols + k];
-            lmin = (d < lmin) ? d : lmin;  // FLT_MAX itself never counts (strict '<')
+        const float *rm = rowmin + b * rh;
+        // lane value: the minimum of its rows; the 8th smallest lane value
+        // bounds the 8th smallest distance from above (8 lanes each own a
+        // candidate at or below it)
+        float a = FLT_MAX;
+        for (int r = lane; r < rh; r += 64) {
+            a = fminf(a, rm[r]);
         }
-        // 8th smallest lane minimum: a valid upper bound of the 8th distance
-        float key = lmin;
-#pragma unroll
-        for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const float o = __shfl_xor(key, j);
-                const bool asc = (lane & k) == 0;
-                const bool lower = (lane & j) == 0;
-                key = (lower == asc) ? fminf(key, o) : fmaxf(key, o);
-            }
-        }
-        const float thr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), 7));
+        const float thr = kth_smallest8(a, lane);
+        // survivors (distance <= thr) of the rows that can hold one, staged in LDS
+        unsigned long long *sk = stage_key + wave * 64;
+        uint32_t *sxy = stage_xy + wave * 64;
         TopList L;
         list_reset(L);
-        for (int q0 = 0; q0 < n; q0 += 64) {
-            const int q = q0 + lane;
-            const int r = q / rw, k = q - r * rw;
-            const bool in = q < n;
-            const float d = in ? db[r * A.dist_cols + k] : FLT_MAX;
-            const uint32_t xy = uint32_t(left + k) | (uint32_t(top + r) << 16);
-            list_offer(L, in && d <= thr, d, q, xy, lane);
+        int staged = 0;
+        bool merged = false;
+        for (int r0 = 0; r0 < rh; r0 += 64) {
+            uint64_t rows = __ballot(r0 + lane < rh && rm[r0 + lane] <= thr);
+            while (rows) {
+                const int r = r0 + __ffsll((unsigned long long)rows) - 1;
+                rows &= rows - 1;
+                for (int k0 = 0; k0 < rw; k0 += 64) {
+                    const int k = k0 + lane;
+                    const float d = k < rw ? db[r * A.dist_cols + k] : FLT_MAX;
+                    const bool surv = k < rw && d <= thr && d < FLT_MAX;
+                    const uint64_t m = __ballot(surv);
+                    const int cnt = __popcll(m);
+                    if (staged + cnt > 64) {
+                        // rare (many ties): fold the stage into the list first
+                        wave_lds_sync();
+                        const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+                        const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+                        wave_lds_sync();
+                        list_merge_batch(L, key, kxy, lane);
+                        staged = 0;
+                        merged = true;
+                    }
+                    if (surv) {
+                        const int pos = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
+                        sk[pos] = make_key(d, r * rw + k);
+                        sxy[pos] = uint32_t(left + k) | (uint32_t(top + r) << 16);
+                    }
+                    staged += cnt;
+                }
+            }
         }
+        wave_lds_sync();
+        {
+            const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+            const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+            if (!merged && staged <= 16) {
+                // the usual case: sort the <= 16 survivors inside one DPP row
+                uint64_t k16 = key;
+                uint32_t x16 = kxy;
+                sort16_dpp(k16, x16, lane);
+                if (lane < 8) {
+                    L.key = k16 < kPlaceholderKey ? k16 : kPlaceholderKey;
+                    L.xy = k16 < kPlaceholderKey ? x16 : 0u;
+                }
+            } else {
+                list_merge_batch(L, key, kxy, lane);
+            }
+        }
+        wave_lds_sync();
         if (A.spatial) {
             if (lane < 8) {
                 A.spatial[(long long)(by * A.nx + bx) * 8 + lane] = make_uint2(uint32_t(L.key >> 32), L.xy);
@@ -1123,9 +1256,7 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
         if (a.radius == 0) {
             rows.spatial = nullptr;
         }
-        const int rh = min(2 * a.bm_range + 1, a.height - 7);
-        const size_t lds = sizeof(float) * (size_t(a.region_cols) * a.region_rows + size_t(a.run_blocks) * 64 +
-                                            size_t(a.run_blocks) * rh * a.dist_cols);
+        const size_t lds = size_t(rows_lds_bytes(a.region_cols, a.region_rows, a.run_blocks, a.dist_cols));
         const dim3 rgrid(a.ny * a.runs_x);
         if (fused) {
             hipLaunchKernelGGL((bm3d_match_rows<kFpAll>), rgrid, block, lds, s, rows);
