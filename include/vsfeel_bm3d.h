@@ -169,6 +169,42 @@ int vsfeel_bm3d_debug_estimate(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
                                float *res, int32_t res_pitch, char *err, size_t err_len);
 
 /*
+ * Differentiable form (no counterpart in the reference; built on the same
+ * kernels).  With the block-match groups and the threshold masks held fixed,
+ * the basic (hard-threshold) filter at radius 0 is out = A x / D with A a
+ * fixed symmetric linear map (per group: gather, 3-D DCT, mask, inverse,
+ * weight, scatter) and D the weight sums, so d(loss)/dx = A (g / D).
+ *
+ * Saved state per output frame i of a batch (device buffers, laid out
+ * back to back for the count frames):
+ *   groups  [count][ref_blocks_y * ref_blocks_x][8] uint32 (packed x | y<<14 | z<<28)
+ *   masks   [count][ref blocks][8] uint64 keep masks (lane u, bit w*8+v)
+ *   weights [count][ref blocks] float
+ *   denom   [count][height][denom_pitch] float weight sums D
+ */
+typedef struct vsfeel_bm3d_saved {
+    uint32_t *groups;
+    uint64_t *masks;
+    float *weights;
+    float *denom;
+    int32_t denom_pitch;
+} vsfeel_bm3d_saved;
+
+/* Forward pass that also fills `saved` (radius 0, basic pass only). */
+int vsfeel_bm3d_forward_saved(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip,
+                              int32_t first, int32_t count,
+                              float *out, int32_t out_pitch, int64_t out_frame_stride,
+                              const vsfeel_bm3d_saved *saved, void *stream,
+                              char *err, size_t err_len);
+
+/* Backward pass: grad_in = A (grad_out / D) per frame, from `saved`. */
+int vsfeel_bm3d_backward(vsfeel_bm3d *h, const float *grad_out, int32_t grad_pitch,
+                         int64_t grad_frame_stride, int32_t count,
+                         const vsfeel_bm3d_saved *saved,
+                         float *grad_in, int32_t gin_pitch, int64_t gin_frame_stride,
+                         void *stream, char *err, size_t err_len);
+
+/*
  * Per-kernel GPU timing, the counterpart of the reference's BM3D_GPUTRACE
  * timestamp queries (src/bm3d.cpp:1492-1501, :978-994): when enabled, HIP
  * events are recorded around every launch on the launch stream.

@@ -67,6 +67,17 @@ struct EstimateArgs {
     int nx, ny, step, halo;
     int tile_bx, tile_by, tiles_x, tiles_y;
     int region_w, region_h;
+    // differentiable form: per group keep masks (8 words: column frequency
+    // lane u, bit w*8+v) and weights, written (kFilterSave) or applied
+    // (kFilterApply)
+    uint64_t *masks;
+    float *weights;
+};
+
+enum FilterMode : int {
+    kFilterPlain = 0,  // threshold the input's own spectrum
+    kFilterSave = 1,   // ... and store the keep masks and weights
+    kFilterApply = 2,  // apply stored masks and weights (a fixed linear map)
 };
 
 // Pixel span [lo, hi] that the groups of tile t (along one axis) can reach in
@@ -103,6 +114,9 @@ struct AggArgs {
     int width, height, pitch;        // res geometry
     int dst_pitch;
     int tw;
+    float *denom;                    // optional: the weight sums (the output's denominator)
+    int denom_pitch;
+    int numerator_only;              // write the weighted sums without dividing
 };
 
 enum FpMode : int {

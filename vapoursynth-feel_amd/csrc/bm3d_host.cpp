@@ -21,7 +21,9 @@
 namespace vsfeel {
 
 hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s);
-hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, hipStream_t s);
+hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, int mode, hipStream_t s);
+hipError_t launch_divide(const float *num, int num_pitch, const float *den, int den_pitch, float *out, int out_pitch,
+                         int width, int height, hipStream_t s);
 hipError_t launch_aggregate(const AggArgs &a, hipStream_t s);
 hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
 hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s);
@@ -96,6 +98,13 @@ bool choose_tiles(int width, int height, int step, int range, int *tbx, int *tby
 }
 
 }  // namespace
+
+// Saved state of the differentiable form for one frame (or none).
+struct SavedState {
+    uint32_t *groups = nullptr;
+    uint64_t *masks = nullptr;
+    float *weights = nullptr;
+};
 
 struct Instance {
     // parameters (BM3DData, src/bm3d.cpp:74-130)
@@ -273,8 +282,14 @@ struct Instance {
         return m;
     }
 
+    using Saved = SavedState;
+
+    // mode kFilterPlain / kFilterSave: block matching into the group table
+    // (saved.groups when given), then filtering; kFilterApply: no matching,
+    // the saved groups, masks and weights are applied to `input`
     hipError_t estimate(const float *const *search, const float *const *input, int in_pitch,
-                        float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr)
+                        float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr,
+                        int mode = kFilterPlain, const Saved &saved = Saved())
     {
         hipError_t e = hipSuccess;
         if (!tiled || radius > 0 || short_windows) {
@@ -287,17 +302,23 @@ struct Instance {
                 return e;
             }
         }
-        const MatchArgs m = match_args(search, in_pitch, dists_out);
-        e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
-        if (e != hipSuccess) {
-            return e;
+        uint32_t *table = saved.groups ? saved.groups : groups;
+        if (mode != kFilterApply) {
+            MatchArgs m = match_args(search, in_pitch, dists_out);
+            m.groups = table;
+            e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
+            if (e != hipSuccess) {
+                return e;
+            }
         }
         EstimateArgs a {};
         for (int z = 0; z < tw; ++z) {
             a.search[z] = search[z];
             a.input[z] = input[z];
         }
-        a.groups = groups;
+        a.groups = table;
+        a.masks = saved.masks;
+        a.weights = saved.weights;
         a.res = stack;
         a.height = height;
         a.pitch = in_pitch;
@@ -320,7 +341,7 @@ struct Instance {
             a.region_w = region_w;
             a.region_h = region_h;
         }
-        e = timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, s); });
+        e = timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, mode, s); });
         if (e != hipSuccess || !tiled) {
             return e;
         }
@@ -342,9 +363,13 @@ struct Instance {
     }
 
     // aggregation of frame n from the res slots (record_bm3d_agg, src/bm3d.cpp:628-696)
-    hipError_t aggregate(int n, const std::vector<int> &slot_of, float *dst, int dst_pitch, hipStream_t s)
+    hipError_t aggregate(int n, const std::vector<int> &slot_of, float *dst, int dst_pitch, hipStream_t s,
+                         float *denom = nullptr, int denom_pitch = 0, bool numerator_only = false)
     {
         AggArgs a {};
+        a.denom = denom;
+        a.denom_pitch = denom_pitch;
+        a.numerator_only = numerator_only ? 1 : 0;
         for (int i = 0; i < tw; ++i) {
             const int z = radius == 0 ? 0 : agg_z(i, n, nframes, radius);
             a.slice[i] = res_slot(slot_of[i]) + (long long)z * 2 * pe;
@@ -868,6 +893,109 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
                    "aggregate launch");
     }
     // the res slots now hold this call's stacks, not the host path's frames
+    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    return VSFEEL_BM3D_OK;
+}
+
+static int check_saved(const Instance &d, const vsfeel_bm3d_saved *saved, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (d.radius != 0 || d.final_pass) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL,
+                    "the differentiable form supports radius 0 and the basic (hard-threshold) pass");
+    }
+    if (!saved || !saved->groups || !saved->masks || !saved->weights || !saved->denom ||
+        saved->denom_pitch < d.width) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "incomplete saved state");
+    }
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_forward_saved(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip, int32_t first, int32_t count,
+                              float *out, int32_t out_pitch, int64_t out_frame_stride, const vsfeel_bm3d_saved *saved,
+                              void *stream, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!h) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null handle");
+    }
+    Instance &d = h->inst;
+    int rc = check_device_clip(d, clip, err, err_len);
+    if (rc == VSFEEL_BM3D_OK) {
+        rc = check_saved(d, saved, err, err_len);
+    }
+    if (rc != VSFEEL_BM3D_OK) {
+        return rc;
+    }
+    if (first < 0 || count < 0 || first + count > d.nframes) {
+        return fail(err, err_len, VSFEEL_BM3D_ERANGE, "frame range outside the clip");
+    }
+    if (!out || out_pitch < d.width) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "bad output buffer");
+    }
+    std::lock_guard<std::mutex> lock(d.mu);
+    VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
+    const std::vector<int> slot_of(1, 0);
+    for (int i = 0; i < count; ++i) {
+        const int n = first + i;
+        const float *search[kMaxWindow];
+        const float *input[kMaxWindow];
+        device_window(d, clip, n, search, input);
+        Instance::Saved sv;
+        sv.groups = saved->groups + (int64_t)i * d.nref * 8;
+        sv.masks = saved->masks + (int64_t)i * d.nref * 8;
+        sv.weights = saved->weights + (int64_t)i * d.nref;
+        VSFEEL_HIP(d.estimate(search, input, clip->pitch, d.res_slot(0), d.pitch, s, nullptr, kFilterSave, sv),
+                   "estimate launch");
+        VSFEEL_HIP(d.aggregate(n, slot_of, out + (int64_t)i * out_frame_stride, out_pitch, s,
+                               saved->denom + (int64_t)i * d.height * saved->denom_pitch, saved->denom_pitch),
+                   "aggregate launch");
+    }
+    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_backward(vsfeel_bm3d *h, const float *grad_out, int32_t grad_pitch, int64_t grad_frame_stride,
+                         int32_t count, const vsfeel_bm3d_saved *saved, float *grad_in, int32_t gin_pitch,
+                         int64_t gin_frame_stride, void *stream, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!h) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null handle");
+    }
+    Instance &d = h->inst;
+    int rc = check_saved(d, saved, err, err_len);
+    if (rc != VSFEEL_BM3D_OK) {
+        return rc;
+    }
+    if (!grad_out || !grad_in || grad_pitch < d.width || gin_pitch < d.width || count < 0) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "bad gradient buffers");
+    }
+    std::lock_guard<std::mutex> lock(d.mu);
+    VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
+    const std::vector<int> slot_of(1, 0);
+    for (int i = 0; i < count; ++i) {
+        // y = g / D, then grad = A y with the saved groups, masks and weights
+        VSFEEL_HIP(launch_divide(grad_out + (int64_t)i * grad_frame_stride, grad_pitch,
+                                 saved->denom + (int64_t)i * d.height * saved->denom_pitch, saved->denom_pitch,
+                                 d.dst_dev, d.pitch, d.width, d.height, s),
+                   "divide launch");
+        const float *y[kMaxWindow] = { d.dst_dev };
+        Instance::Saved sv;
+        sv.groups = saved->groups + (int64_t)i * d.nref * 8;
+        sv.masks = saved->masks + (int64_t)i * d.nref * 8;
+        sv.weights = saved->weights + (int64_t)i * d.nref;
+        VSFEEL_HIP(d.estimate(y, y, d.pitch, d.res_slot(0), d.pitch, s, nullptr, kFilterApply, sv),
+                   "estimate launch");
+        VSFEEL_HIP(d.aggregate(0, slot_of, grad_in + (int64_t)i * gin_frame_stride, gin_pitch, s, nullptr, 0, true),
+                   "aggregate launch");
+    }
     std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
     return VSFEEL_BM3D_OK;

@@ -941,9 +941,15 @@ __device__ __forceinline__ float group8_sum_tree(float k)
 // inverse (collaborative_hard / collaborative_wiener, src/bm3d.comp:384-443).
 // On return d[i*8 + j] is the filtered block i, row j, column c and the
 // result is the group's adaptive weight.
-template <int CM, bool FINAL>
+//
+// MODE kFilterSave also stores the group's keep mask (one 64-bit word per
+// column-frequency lane, bit w*8+v) and its weight; MODE kFilterApply uses a
+// stored mask and weight instead of thresholding the input's spectrum, which
+// makes the whole filter a fixed linear map of its input (the backward pass
+// of the hard-threshold filter: the map is symmetric, C^T M C per group).
+template <int CM, bool FINAL, int MODE>
 __device__ __forceinline__ float filter_group(const EstimateArgs &A, const uint32_t *ent, float *tb, int c,
-                                              float *d)
+                                              float *d, int gid, bool active)
 {
     // gather: d[i*8 + j] = block i, row j, column c (src/bm3d.comp:677-689)
     float rf[FINAL ? 64 : 1];
@@ -979,15 +985,24 @@ __device__ __forceinline__ float filter_group(const EstimateArgs &A, const uint3
     // lane c now holds column frequency u = c, d[w*8 + v]
     const float scale = 1.0f / 4096.0f;
     float ks[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    uint64_t mask = 0;
+    if (MODE == kFilterApply) {
+        mask = A.masks[(long long)gid * 8 + c];
+    }
 #pragma unroll
     for (int i = 0; i < 64; ++i) {
         const float val = d[i];
-        if (!FINAL) {
+        if (MODE == kFilterApply) {
+            d[i] = ((mask >> i) & 1ull) ? val * scale : 0.0f;
+        } else if (!FINAL) {
             // hard threshold, the DC term always kept (src/bm3d.comp:358-382)
             const float thr = (i == 0 && c == 0) ? 0.0f : A.sigma;
             const bool keep = fabsf(val) >= thr;
             ks[i % 4] += keep ? 1.0f : 0.0f;
             d[i] = keep ? val * scale : 0.0f;
+            if (MODE == kFilterSave) {
+                mask |= uint64_t(keep) << i;
+            }
         } else {
             // Wiener shrinkage against the reference's spectrum (src/bm3d.comp:403-420)
             const float rv = rf[i];
@@ -1002,9 +1017,20 @@ __device__ __forceinline__ float filter_group(const EstimateArgs &A, const uint3
             d[i] = filtered * scale;
         }
     }
-    const float kl = (ks[0] + ks[1]) + (ks[2] + ks[3]);
-    const float k = group8_sum_tree(kl);
-    const float w = 1.0f / k;
+    float w;
+    if (MODE == kFilterApply) {
+        w = A.weights[gid];
+    } else {
+        const float kl = (ks[0] + ks[1]) + (ks[2] + ks[3]);
+        const float k = group8_sum_tree(kl);
+        w = 1.0f / k;
+    }
+    if (MODE == kFilterSave && active) {
+        A.masks[(long long)gid * 8 + c] = mask;
+        if (c == 0) {
+            A.weights[gid] = w;
+        }
+    }
 
     // inverse: vertical, horizontal, group axis
     transform_rows<CM, true>(d);
@@ -1034,7 +1060,7 @@ __device__ __forceinline__ void scatter_global(const EstimateArgs &A, uint32_t e
 
 // Fallback for search ranges whose tile accumulator would not fit in LDS:
 // one group per 8 lanes in raster order, every contribution a global atomic.
-template <int CM, bool FINAL>
+template <int CM, bool FINAL, int MODE>
 __global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
 {
     __shared__ float tile[4][8][72];
@@ -1055,7 +1081,7 @@ __global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
         ent[i] = A.groups[(long long)gsafe * 8 + i];
     }
     float d[64];
-    const float w = filter_group<CM, FINAL>(A, ent, tb, c, d);
+    const float w = filter_group<CM, FINAL, MODE>(A, ent, tb, c, d, gsafe, active);
     if (active) {
         const float wq = (w + A.extractor) - A.extractor;
 #pragma unroll
@@ -1100,7 +1126,7 @@ constexpr int kTiledGroupsPerPass = kTiledWaves * 8;
 // slot of the scratch buffer; the bm3d_tile_reduce pass adds the overlapping
 // tiles in a fixed order.  Blocks in the other frames of the temporal window
 // go to the dense slices with global atomics.
-template <int CM, bool FINAL>
+template <int CM, bool FINAL, int MODE>
 __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArgs A)
 {
     extern __shared__ __align__(16) unsigned long long acc[];  // [2][region_h][region_w]
@@ -1140,7 +1166,7 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
             ent[i] = A.groups[(long long)gid * 8 + i];
         }
         float d[64];
-        const float w = filter_group<CM, FINAL>(A, ent, tb, c, d);
+        const float w = filter_group<CM, FINAL, MODE>(A, ent, tb, c, d, gid, active);
         if (active) {
             const float E = A.extractor;
             const float wq = (w + E) - E;
@@ -1227,7 +1253,23 @@ __global__ __launch_bounds__(256) void bm3d_aggregate(AggArgs A)
         acc += (double)A.slice[i][q];
         acw += (double)A.slice[i][q + A.plane_elems];
     }
-    A.dst[(long long)y * A.dst_pitch + x] = (float)(acc / acw);
+    A.dst[(long long)y * A.dst_pitch + x] = A.numerator_only ? (float)acc : (float)(acc / acw);
+    if (A.denom) {
+        A.denom[(long long)y * A.denom_pitch + x] = (float)acw;
+    }
+}
+
+// out = num / den elementwise (the backward pass's scaling of the upstream
+// gradient by the output's denominator)
+__global__ __launch_bounds__(256) void bm3d_divide(const float *num, int num_pitch, const float *den, int den_pitch,
+                                                   float *out, int out_pitch, int width, int height)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= width || y >= height) {
+        return;
+    }
+    out[(long long)y * out_pitch + x] = num[(long long)y * num_pitch + x] / den[(long long)y * den_pitch + x];
 }
 
 __global__ __launch_bounds__(256) void fill_zero(float4 *p, long long n4)
@@ -1291,31 +1333,44 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, hipStream_t s)
+hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, int mode, hipStream_t s)
 {
     const bool tiled = a.scratch != nullptr;
     const dim3 grid(tiled ? a.tiles_x * a.tiles_y : (a.ngroups + 31) / 32), block(tiled ? kTiledThreads : 256);
     const size_t lds = tiled ? size_t(2) * a.region_h * a.region_w * sizeof(unsigned long long) : 0;
-#define VSFEEL_EST(CM)                                                                         \
+#define VSFEEL_EST(CM, FIN, MODE)                                                              \
     if (tiled) {                                                                               \
-        if (final_pass) {                                                                      \
-            hipLaunchKernelGGL((bm3d_estimate_tiled<CM, true>), grid, block, lds, s, a);       \
-        } else {                                                                               \
-            hipLaunchKernelGGL((bm3d_estimate_tiled<CM, false>), grid, block, lds, s, a);      \
-        }                                                                                      \
-    } else if (final_pass) {                                                                   \
-        hipLaunchKernelGGL((bm3d_estimate<CM, true>), grid, block, 0, s, a);                   \
+        hipLaunchKernelGGL((bm3d_estimate_tiled<CM, FIN, MODE>), grid, block, lds, s, a);      \
     } else {                                                                                   \
-        hipLaunchKernelGGL((bm3d_estimate<CM, false>), grid, block, 0, s, a);                  \
+        hipLaunchKernelGGL((bm3d_estimate<CM, FIN, MODE>), grid, block, 0, s, a);              \
+    }
+#define VSFEEL_EST_CM(CM)                                                                      \
+    if (final_pass) {                                                                          \
+        VSFEEL_EST(CM, true, kFilterPlain)                                                     \
+    } else if (mode == kFilterSave) {                                                          \
+        VSFEEL_EST(CM, false, kFilterSave)                                                     \
+    } else if (mode == kFilterApply) {                                                         \
+        VSFEEL_EST(CM, false, kFilterApply)                                                    \
+    } else {                                                                                   \
+        VSFEEL_EST(CM, false, kFilterPlain)                                                    \
     }
     if (fp_mode == kFpNone) {
-        VSFEEL_EST(kFpNone)
+        VSFEEL_EST_CM(kFpNone)
     } else if (fp_mode == kFpAll) {
-        VSFEEL_EST(kFpAll)
+        VSFEEL_EST_CM(kFpAll)
     } else {
-        VSFEEL_EST(kFpExplicit)
+        VSFEEL_EST_CM(kFpExplicit)
     }
+#undef VSFEEL_EST_CM
 #undef VSFEEL_EST
+    return hipGetLastError();
+}
+
+hipError_t launch_divide(const float *num, int num_pitch, const float *den, int den_pitch, float *out, int out_pitch,
+                         int width, int height, hipStream_t s)
+{
+    const dim3 grid((width + 63) / 64, (height + 3) / 4), block(256);
+    hipLaunchKernelGGL(bm3d_divide, grid, block, 0, s, num, num_pitch, den, den_pitch, out, out_pitch, width, height);
     return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ EXPORTS = (
     "vsfeel_bm3d_debug_estimate",
     "vsfeel_bm3d_set_timing",
     "vsfeel_bm3d_kernel_times",
+    "vsfeel_bm3d_forward_saved",
+    "vsfeel_bm3d_backward",
 )
 
 KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce")
@@ -91,6 +93,16 @@ class DeviceClip(ctypes.Structure):
     ]
 
 
+class Saved(ctypes.Structure):
+    _fields_ = [
+        ("groups", ctypes.c_void_p),
+        ("masks", ctypes.c_void_p),
+        ("weights", ctypes.c_void_p),
+        ("denom", ctypes.c_void_p),
+        ("denom_pitch", ctypes.c_int32),
+    ]
+
+
 _lib = None
 
 
@@ -132,6 +144,12 @@ def load() -> ctypes.CDLL:
     L.vsfeel_bm3d_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_int64)]
     L.vsfeel_bm3d_kernel_times.restype = i32
+    L.vsfeel_bm3d_forward_saved.argtypes = [vp, ctypes.POINTER(DeviceClip), i32, i32, vp, i32,
+                                            ctypes.c_int64, ctypes.POINTER(Saved), vp, cp, sz]
+    L.vsfeel_bm3d_forward_saved.restype = i32
+    L.vsfeel_bm3d_backward.argtypes = [vp, vp, i32, ctypes.c_int64, i32, ctypes.POINTER(Saved), vp, i32,
+                                       ctypes.c_int64, vp, cp, sz]
+    L.vsfeel_bm3d_backward.restype = i32
     _lib = L
     return L
 

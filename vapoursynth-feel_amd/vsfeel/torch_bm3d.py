@@ -101,6 +101,59 @@ class Instance:
             raise Error(err.value.decode())
         return out
 
+    # ------------------------------------------------------- differentiable form
+    def forward_saved(self, x: torch.Tensor):
+        """Forward pass that also returns the state its backward needs:
+        (out, (groups int32 [N, nref, 8], masks int64 [N, nref, 8],
+        weights float32 [N, nref], denom float32 [N, H, W]))."""
+        n = self.num_frames
+        nref = self.info["ref_blocks_x"] * self.info["ref_blocks_y"]
+        dev = x.device
+        groups = torch.empty((n, nref, 8), dtype=torch.int32, device=dev)
+        masks = torch.empty((n, nref, 8), dtype=torch.int64, device=dev)
+        weights = torch.empty((n, nref), dtype=torch.float32, device=dev)
+        denom = torch.empty((n, self.height, self.width), dtype=torch.float32, device=dev)
+        out = torch.empty((n, self.height, self.width), dtype=torch.float32, device=dev)
+        c = self._clip(x, None)
+        sv = self._saved(groups, masks, weights, denom)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        err = _lib.err_buf()
+        rc = _lib.load().vsfeel_bm3d_forward_saved(
+            self.handle, ctypes.byref(c), 0, n, out.data_ptr(), out.stride(1), out.stride(0),
+            ctypes.byref(sv), ctypes.c_void_p(stream), err, len(err))
+        if rc != _lib.OK:
+            raise Error(err.value.decode())
+        return out, (groups, masks, weights, denom)
+
+    def backward(self, grad_out: torch.Tensor, saved) -> torch.Tensor:
+        """d(loss)/dx = A (grad_out / denom) with the saved groups, masks, weights."""
+        groups, masks, weights, denom = saved
+        g = grad_out.contiguous()
+        _check(g, self)
+        gin = torch.empty_like(g)
+        sv = self._saved(groups, masks, weights, denom)
+        stream = torch.cuda.current_stream(g.device).cuda_stream
+        err = _lib.err_buf()
+        rc = _lib.load().vsfeel_bm3d_backward(
+            self.handle, g.data_ptr(), g.stride(1), g.stride(0), self.num_frames, ctypes.byref(sv),
+            gin.data_ptr(), gin.stride(1), gin.stride(0), ctypes.c_void_p(stream), err, len(err))
+        if rc != _lib.OK:
+            raise Error(err.value.decode())
+        return gin
+
+    @staticmethod
+    def _saved(groups, masks, weights, denom) -> _lib.Saved:
+        for t in (groups, masks, weights, denom):
+            if not t.is_contiguous():
+                raise Error("BM3D: saved state must be contiguous")
+        sv = _lib.Saved()
+        sv.groups = groups.data_ptr()
+        sv.masks = masks.data_ptr()
+        sv.weights = weights.data_ptr()
+        sv.denom = denom.data_ptr()
+        sv.denom_pitch = denom.stride(1)
+        return sv
+
     def set_timing(self, enable: bool) -> None:
         """Record HIP events around every kernel launch (resets the totals)."""
         _lib.load().vsfeel_bm3d_set_timing(self.handle, int(bool(enable)))
