@@ -38,14 +38,15 @@ struct MatchArgs {
 };
 
 // LDS bytes of bm3d_match_rows: survivor stages of its 4 wavefronts (64 keys
-// + positions each), the source region, the blocks, their distance rows and
-// row minima (region_rows - 7 window rows at most).
+// + positions each), per window row its 3 smallest distances and the
+// positions of the first two (16 bytes), the source region, the blocks, their
+// distance rows (region_rows - 7 window rows at most) and one overflow flag
+// per block.
 __host__ __device__ inline long long rows_lds_bytes(int region_cols, int region_rows, int run_blocks, int dist_cols)
 {
     const long long rh = region_rows - 7;
-    return 256LL * 8 + 256LL * 4 +
-           4LL * ((long long)region_cols * region_rows + run_blocks * 64LL + run_blocks * rh * dist_cols +
-                  run_blocks * rh);
+    return 256LL * 8 + 256LL * 4 + 16LL * run_blocks * rh +
+           4LL * ((long long)region_cols * region_rows + run_blocks * 64LL + run_blocks * rh * dist_cols + run_blocks);
 }
 
 // Collaborative filtering + weighted scatter of every group.

@@ -571,7 +571,8 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
         // one window row per thread of a 256-thread workgroup; the source
         // region, the blocks and their distance rows must fit 64 KB of LDS
         const int rh = std::min(2 * d.bm_range + 1, d.height - 7);
-        const int nb = rh <= 256 ? 256 / rh : 0;
+        // (the 16-lane selection keeps a 64-bit mask of the window rows)
+        const int nb = rh <= 64 ? 256 / rh : 0;
         if (nb > 0) {
             const int cols = std::min((nb - 1) * d.block_step + 8 + 2 * d.bm_range, d.width);
             const int dcols = std::min(2 * d.bm_range + 1, d.width - 7);

@@ -362,6 +362,33 @@ __device__ __forceinline__ void finish_group(const MatchArgs &A, GroupList &G, i
     }
 }
 
+// finish_group for a list held by lanes sub 0..7 of a 16-lane row (the
+// spatial search at radius 0, every member in the frame itself)
+__device__ __forceinline__ void finish_group16(const MatchArgs &A, float e, uint32_t xy, int blk, int x, int y,
+                                               int sub)
+{
+    const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
+    const int shift = (lane_id() >> 4) * 16;
+    const uint32_t seg = uint32_t(__ballot(sub < 8 && xy == self_xy) >> shift) & 0xFFu;
+    if (seg == 0) {
+        const uint32_t up_xy = dpp_row_shr1(xy);
+        const float up_e = __uint_as_float(dpp_row_shr1(__float_as_uint(e)));
+        if (sub == 0) {
+            e = -1.0f;
+            xy = self_xy;
+        } else if (sub < 8) {
+            e = up_e;
+            xy = up_xy;
+        }
+    }
+    if (sub < 8) {
+        A.groups[(long long)blk * 8 + sub] = pack_entry(xy & 0xFFFFu, xy >> 16, uint32_t(A.radius));
+        if (A.dists) {
+            A.dists[(long long)blk * 8 + sub] = e;
+        }
+    }
+}
+
 // DPP move of v; lanes without a source lane in their row keep `old`
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t v)
@@ -421,6 +448,26 @@ __device__ __forceinline__ uint32_t dpp_xor(uint32_t v, int lane)
     }
 }
 
+// minimum over the 16 lanes of each row, in every lane of the row
+__device__ __forceinline__ float row_min16(float v, int lane)
+{
+    v = fminf(v, __uint_as_float(dpp_xor<1>(__float_as_uint(v), lane)));
+    v = fminf(v, __uint_as_float(dpp_xor<2>(__float_as_uint(v), lane)));
+    v = fminf(v, __uint_as_float(dpp_xor<4>(__float_as_uint(v), lane)));
+    v = fminf(v, __uint_as_float(dpp_xor<8>(__float_as_uint(v), lane)));
+    return v;
+}
+
+// inclusive prefix sum over the 16 lanes of each row (row_shr with zero fill)
+__device__ __forceinline__ int row_prefix_sum16(int v)
+{
+    v += int(dpp_u32<0x111>(0u, uint32_t(v)));  // row_shr:1
+    v += int(dpp_u32<0x112>(0u, uint32_t(v)));  // row_shr:2
+    v += int(dpp_u32<0x114>(0u, uint32_t(v)));  // row_shr:4
+    v += int(dpp_u32<0x118>(0u, uint32_t(v)));  // row_shr:8
+    return v;
+}
+
 template <int K, int J>
 __device__ __forceinline__ void sort16_step(uint64_t &key, uint32_t &xy, int lane)
 {
@@ -459,10 +506,12 @@ __device__ __forceinline__ void sort16_dpp(uint64_t &key, uint32_t &xy, int lane
 // serves them all.  Phase 1: every thread owns one (block, window row) and
 // walks the row: column SSDs v_j against reference column min(j, 7), then
 // D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
-// written to the block's distance rows in LDS.  Phase 2: one wavefront per
-// block picks the top-8 (distance, seq) keys: the 8th smallest of the 64
-// lane minima bounds the 8th smallest distance, and only candidates at or
-// below it are offered to the list.
+// written to the block's distance rows in LDS, while it keeps the row's 3
+// smallest distances.  Phase 2: one 16-lane row of a wavefront per block
+// takes the 8th smallest of those stored distances as a threshold (it bounds
+// the window's 8th smallest distance from above), gathers the candidates at
+// or below it (about 9 on natural images) and sorts them; blocks with more
+// than 16 such candidates go to a whole-wavefront pass.
 // ---------------------------------------------------------------------------
 template <int CM>
 __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
@@ -473,10 +522,13 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
     const int RH = A.region_rows - 7;
     unsigned long long *stage_key = lds64;                                  // [4][64] survivors
     uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);         // [4][64]
-    float *region = reinterpret_cast<float *>(stage_xy + 256);              // [region_cols][region_rows]
+    // per window row: its 3 smallest distances (x <= y <= z) and the
+    // positions of the first two (w = k1 | k2 << 16)
+    float4 *rowtop = reinterpret_cast<float4 *>(stage_xy + 256);            // [NB][rh]
+    float *region = reinterpret_cast<float *>(rowtop + NB * RH);            // [region_cols][region_rows]
     float *cur = region + A.region_cols * CS;                               // [NB][64]
     float *dist = cur + NB * 64;                                            // [NB][rh][dist_cols]
-    float *rowmin = dist + NB * RH * A.dist_cols;                           // [NB][rh]
+    int *stage_over = reinterpret_cast<int *>(dist + NB * RH * A.dist_cols);  // [NB] more than 16 survivors
 
     const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
     const int run = xcd_block_id();
@@ -520,7 +572,8 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         }
         const float *col0 = region + (left - rx0) * CS + r;  // column j at col0 + j*CS
         float *drow = dist + (b * rh + r) * A.dist_cols;
-        float rmin = FLT_MAX;
+        float m1 = FLT_MAX, m2 = FLT_MAX, m3 = FLT_MAX;  // the row's 3 smallest distances
+        uint32_t k1 = 0, k2 = 0;                          // positions of m1, m2
         float v[15];
         for (int k0 = 0; k0 < rw; k0 += 8) {
             if (k0 == 0) {
@@ -581,31 +634,187 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
                 if (k0 + i < rw) {
                     const float d = p2[i] + p2[i + 4];
                     drow[k0 + i] = d;
-                    rmin = (d < rmin) ? d : rmin;  // FLT_MAX and NaN never count
+                    // FLT_MAX and NaN never count (the shader's strict '<')
+                    const float dv = (d < FLT_MAX) ? d : FLT_MAX;
+                    const bool c1 = dv < m1, c2 = dv < m2;
+                    k2 = c1 ? k1 : (c2 ? uint32_t(k0 + i) : k2);
+                    k1 = c1 ? uint32_t(k0 + i) : k1;
+                    const float t1 = fmaxf(m1, dv);
+                    m1 = fminf(m1, dv);
+                    const float t2 = fmaxf(m2, t1);
+                    m2 = fminf(m2, t1);
+                    m3 = fminf(m3, t2);
                 }
             }
         }
-        rowmin[b * rh + r] = rmin;
+        rowtop[b * rh + r] = make_float4(m1, m2, m3, __uint_as_float(k1 | (k2 << 16)));
     }
     __syncthreads();
 
-    // phase 2: top-8 of each block by one wavefront
+    // phase 2: top-8 of each block by one 16-lane row of a wavefront (4
+    // blocks per wavefront at once, all cross-lane work inside DPP rows)
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
+    for (int b0 = 0; b0 < nb; b0 += 16) {
+        const int grp = lane >> 4, sub = lane & 15;
+        const int b = b0 + wave + 4 * grp;
+        const bool act = b < nb;
+        const int bs = act ? b : 0;
+        const int bx = bx0 + bs;
+        const int xb = min(bx * s, W - 8);
+        const int left = max(xb - R, 0), right = min(xb + R, W - 8);
+        const int rw = right - left + 1;
+        const float *db = dist + bs * rh * A.dist_cols;
+        const float4 *rt = rowtop + bs * rh;
+        // threshold: the 8th smallest (ties counted) of the rows' stored 3
+        // smallest distances.  They are distances of the window, so it bounds
+        // the window's 8th smallest distance from above.  Each lane keeps the
+        // sorted 8 smallest of its rows (sub, sub+16, ...) and the row takes
+        // the 8 smallest heads in turn.
+        float h[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            h[i] = FLT_MAX;
+        }
+        for (int r = sub; r < rh; r += 16) {
+            const float4 q = rt[r];
+            const float qv[3] = {q.x, q.y, q.z};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float v = qv[j];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float lo = fminf(h[i], v);
+                    v = fmaxf(h[i], v);
+                    h[i] = lo;
+                }
+            }
+        }
+        float thr = FLT_MAX;
+        int taken = 0;
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const float m = row_min16(h[0], lane);
+            const bool hit = h[0] == m;
+            const uint32_t seg = uint32_t(__ballot(hit) >> (grp * 16)) & 0xFFFFu;
+            if (taken < 8) {
+                taken += __popc(seg);
+                thr = m;
+            }
+            if (hit) {
+#pragma unroll
+                for (int i = 0; i < 7; ++i) {
+                    h[i] = h[i + 1];
+                }
+                h[7] = FLT_MAX;
+            }
+        }
+        // survivors (distance <= thr): a row whose third smallest is above
+        // thr holds them among its stored first two; the other rows (usually
+        // the ref block's own row and a neighbour) are scanned
+        unsigned long long *sk = stage_key + wave * 64 + grp * 16;
+        uint32_t *sxy = stage_xy + wave * 64 + grp * 16;
+        uint64_t rows = 0;
+        uint32_t own = 0;  // stored survivors of this lane's rows, 2 bits per row slot
+        int cnt = 0;
+        for (int r0 = 0; r0 < rh; r0 += 16) {
+            const int r = r0 + sub;
+            bool scan = false;
+            if (act && r < rh) {
+                const float4 q = rt[r];
+                scan = q.z <= thr && q.z < FLT_MAX;
+                if (!scan) {
+                    const uint32_t s0 = (q.x <= thr && q.x < FLT_MAX) ? 1u : 0u;
+                    const uint32_t s1 = (q.y <= thr && q.y < FLT_MAX) ? 1u : 0u;
+                    own |= (s0 | (s1 << 1)) << (r0 >> 3);
+                    cnt += int(s0 + s1);
+                }
+            }
+            const uint64_t m = __ballot(scan);
+            rows |= ((m >> (grp * 16)) & 0xFFFFull) << r0;
+        }
+        const int incl = row_prefix_sum16(cnt);
+        int pos = incl - cnt;
+        const int stored = __shfl(incl, (lane & ~15) | 15);
+        for (int r0 = 0; own != 0; r0 += 16, own >>= 2) {
+            if ((own & 3u) == 0) {
+                continue;
+            }
+            const int r = r0 + sub;
+            const float4 q = rt[r];
+            const uint32_t kk = __float_as_uint(q.w);
+            if ((own & 1u) && pos < 16) {
+                const uint32_t k = kk & 0xFFFFu;
+                sk[pos] = make_key(q.x, r * rw + int(k));
+                sxy[pos] = uint32_t(left + int(k)) | (uint32_t(top + r) << 16);
+            }
+            pos += int(own & 1u);
+            if ((own & 2u) && pos < 16) {
+                const uint32_t k = kk >> 16;
+                sk[pos] = make_key(q.y, r * rw + int(k));
+                sxy[pos] = uint32_t(left + int(k)) | (uint32_t(top + r) << 16);
+            }
+            pos += int((own >> 1) & 1u);
+        }
+        int staged = stored;
+        while (rows) {
+            const int r = __ffsll((unsigned long long)rows) - 1;
+            rows &= rows - 1;
+            for (int k0 = 0; k0 < rw; k0 += 16) {
+                const int k = k0 + sub;
+                const float d = k < rw ? db[r * A.dist_cols + k] : FLT_MAX;
+                const bool surv = k < rw && d <= thr && d < FLT_MAX;
+                const uint32_t seg = uint32_t(__ballot(surv) >> (grp * 16)) & 0xFFFFu;
+                const int pos = staged + __popc(seg & ((1u << sub) - 1u));
+                if (surv && pos < 16) {
+                    sk[pos] = make_key(d, r * rw + k);
+                    sxy[pos] = uint32_t(left + k) | (uint32_t(top + r) << 16);
+                }
+                staged += __popc(seg);
+            }
+        }
+        wave_lds_sync();
+        uint64_t key = sub < staged && sub < 16 ? sk[sub] : kNoKey;
+        uint32_t kxy = sub < staged && sub < 16 ? sxy[sub] : 0u;
+        wave_lds_sync();
+        if (act && staged <= 16) {
+            sort16_dpp(key, kxy, lane);
+            if (!(key < kPlaceholderKey)) {
+                key = kPlaceholderKey;  // fewer than 8 candidates in the window
+                kxy = 0u;
+            }
+            if (A.spatial) {
+                if (sub < 8) {
+                    A.spatial[(long long)(by * A.nx + bx) * 8 + sub] = make_uint2(uint32_t(key >> 32), kxy);
+                }
+            } else {
+                finish_group16(A, __uint_as_float(uint32_t(key >> 32)), kxy, by * A.nx + bx, xb, y, sub);
+            }
+        }
+        if (act && sub == 0) {
+            stage_over[b] = staged > 16 ? 1 : 0;
+        }
+    }
+    __syncthreads();
+
+    // blocks with more than 16 survivors (ties): a whole wavefront each
     for (int b = wave; b < nb; b += 4) {
+        if (!stage_over[b]) {
+            continue;
+        }
         const int bx = bx0 + b;
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         const int rw = right - left + 1;
         const int n = rw * rh;
         const float *db = dist + b * rh * A.dist_cols;
-        const float *rm = rowmin + b * rh;
+        const float4 *rt = rowtop + b * rh;
         // lane value: the minimum of its rows; the 8th smallest lane value
         // bounds the 8th smallest distance from above (8 lanes each own a
         // candidate at or below it)
         float a = FLT_MAX;
         for (int r = lane; r < rh; r += 64) {
-            a = fminf(a, rm[r]);
+            a = fminf(a, rt[r].x);
         }
         const float thr = kth_smallest8(a, lane);
         // survivors (distance <= thr) of the rows that can hold one, staged in LDS
@@ -616,7 +825,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         int staged = 0;
         bool merged = false;
         for (int r0 = 0; r0 < rh; r0 += 64) {
-            uint64_t rows = __ballot(r0 + lane < rh && rm[r0 + lane] <= thr);
+            uint64_t rows = __ballot(r0 + lane < rh && rt[r0 + lane].x <= thr);
             while (rows) {
                 const int r = r0 + __ffsll((unsigned long long)rows) - 1;
                 rows &= rows - 1;
