@@ -11,7 +11,18 @@ duration gives the roofline fraction.  The CPU baseline is the C restatement
 of the reference shaders (the oracle) timed on a bounded sample of the same
 workload on the host cores.
 
+Beside the headline line's `value`, the same JSON line carries the other
+forms, each timed the same way (`forms`):
+
+* ``fwd_bwd``: the differentiable form, 16 frames of 512x512 (sigma 3,
+  block_step 4, bm_range 9, radius 0): forward with saved groups, masks and
+  weights, then the backward pass, frames/s of forward+backward;
+* ``temporal_r2``: 32 frames of 1920x1080 luma shifted by (1.5t, 0.75t) px,
+  radius 2, ps_num 2, ps_range 7, extractor_exp 2.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
+                    [--forms plain,fwd_bwd,temporal_r2] [--cpu-seconds S]
+                    [--copy-calibration-mb MB]
 """
 
 from __future__ import annotations
@@ -19,6 +30,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -31,6 +43,7 @@ sys.path.insert(0, ROOT)
 METRIC = "BM3Dv2 frames/sec at 1080p GrayS (plain vs MFMA-distance), fwd+bwd fps, %peak"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "bm3d_match_1080p_hbm_traffic.md")
 
 
 def ref_positions(w, h, step):
@@ -55,15 +68,134 @@ def matching_flops(w, h, step, bm_range):
 TRANSFORM_FLOP_PER_GROUP = 192 * 44 + 192 * 51 + 512 * 3  # fwd + inv DCTs + threshold
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (its line
+    'traffic_bytes_per_launch <kernel> <workload> <bytes>'), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\S+)\s+([0-9.eE+]+)", line)
+                if m and m.group(1) == kernel and m.group(2) == workload:
+                    return float(m.group(3))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mb):
+    """A plain device copy of `mb` MiB (read mb, write mb), three launches,
+    so that a counter run sees the FETCH_SIZE / WRITE_SIZE of known traffic."""
+    import torch
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+
+
+def timed(inst, run, steps, warmup):
+    """Warm up, then time `steps` calls of run() with the library's per-kernel
+    HIP events on; returns (seconds, {kernel: (avg ms, launches)})."""
+    import torch
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    inst.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    times = inst.kernel_times()
+    inst.set_timing(False)
+    per_kernel = {k: (ms / cnt, cnt) for k, (ms, cnt) in times.items() if cnt > 0}
+    return t1 - t0, per_kernel
+
+
+def roofline(per_kernel, w, h, kw, workload):
+    n_ref = len(ref_positions(w, h, kw["block_step"])[0]) * len(ref_positions(w, h, kw["block_step"])[1])
+    dominant = max(per_kernel, key=lambda k: per_kernel[k][0] * per_kernel[k][1])
+    avg_ms = per_kernel[dominant][0]
+    tw = 2 * kw.get("radius", 0) + 1
+    traffic = measured_traffic(dominant, workload)
+    if dominant == "bm3d_match":
+        work = matching_flops(w, h, kw["block_step"], kw["bm_range"])
+        achieved = work / (avg_ms * 1e-3) / 1e12
+        return dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic, kernel=dominant,
+                    avg_launch_ms=avg_ms, work_per_launch=work,
+                    work_unit="FLOP (spatial matching, reference count; FP32 vector peak)")
+    if dominant == "bm3d_estimate":
+        work = n_ref * TRANSFORM_FLOP_PER_GROUP
+        achieved = work / (avg_ms * 1e-3) / 1e12
+        return dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic, kernel=dominant,
+                    avg_launch_ms=avg_ms, work_per_launch=work,
+                    work_unit="FLOP (3-D transforms + threshold; FP32 vector peak)")
+    work = (2 * tw + 1) * w * h * 4.0
+    achieved = work / (avg_ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=achieved / PEAK_HBM_GBS, traffic=traffic, kernel=dominant,
+                avg_launch_ms=avg_ms, work_per_launch=work, work_unit="bytes")
+
+
+def describe(config, clip_shape, kw, form):
+    nf, h, w = clip_shape
+    return (f"config {config}: {nf} x {w}x{h} GrayS float32, {form}, "
+            + ", ".join(f"{k}={v}" for k, v in kw.items()))
+
+
+def bench_plain(config, steps, warmup, frames=None):
+    import torch
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+    spec = synth.WORKLOADS[config]
+    kw = dict(spec["args"])
+    planes = synth.workload(config, frames=frames)
+    clip = planes[0] if isinstance(planes, tuple) else planes
+    nf, h, w = clip.shape
+    x = torch.from_numpy(clip).to("cuda:0")
+    inst = Instance(nf, h, w, **kw)
+    out = torch.empty_like(x)
+    secs, per_kernel = timed(inst, lambda: inst.process(x, out=out), steps, warmup)
+    return clip, kw, secs, per_kernel
+
+
+def bench_fwd_bwd(steps, warmup):
+    import torch
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+    kw = dict(synth.WORKLOADS[5]["args"])
+    clip = synth.workload(5)
+    nf, h, w = clip.shape
+    x = torch.from_numpy(clip).to("cuda:0")
+    g = torch.from_numpy(np.random.default_rng(5).standard_normal(clip.shape).astype(np.float32)).to("cuda:0")
+    inst = Instance(nf, h, w, **kw)
+
+    def run():
+        _, saved = inst.forward_saved(x)
+        inst.backward(g, saved)
+
+    secs, per_kernel = timed(inst, run, steps, warmup)
+    return clip, kw, secs, per_kernel
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--frames", type=int, default=None, help="override the clip length")
+    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2",
+                    help="comma list of forms to time (plain is the headline value)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-time budget of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run a plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -71,64 +203,28 @@ def main():
               "(see DESIGN.md) but no multi-GPU run is part of it", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    from vsfeel import synth
-    from vsfeel.torch_bm3d import Instance
+    forms = [f for f in args.forms.split(",") if f]
+    if args.copy_calibration_mb > 0:
+        copy_calibration(args.copy_calibration_mb)
 
-    spec = synth.WORKLOADS[args.config]
-    kw = dict(spec["args"])
-    planes = synth.workload(args.config, frames=args.frames)
-    clip = planes[0] if isinstance(planes, tuple) else planes
+    clip, kw, secs, per_kernel = bench_plain(args.config, args.steps, args.warmup, args.frames)
     nf, h, w = clip.shape
-    dev = torch.device("cuda:0")
-    x = torch.from_numpy(clip).to(dev)
-    inst = Instance(nf, h, w, **kw)
-    out = torch.empty_like(x)
-    stream = torch.cuda.current_stream(dev)
+    fps = nf * args.steps / secs
+    workload = f"{w}x{h}_step{kw['block_step']}_range{kw['bm_range']}_r{kw.get('radius', 0)}"
+    roof = roofline(per_kernel, w, h, kw, workload)
 
-    for _ in range(args.warmup):
-        inst.process(x, out=out)
-    torch.cuda.synchronize()
-
-    inst.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        inst.process(x, out=out)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    times = inst.kernel_times()
-    inst.set_timing(False)
-    stream.synchronize()
-
-    elapsed = t1 - t0
-    fps = nf * args.steps / elapsed
-    ms_per_step = elapsed * 1e3 / args.steps
-
-    # roofline of the dominant kernel
-    n_ref = len(ref_positions(w, h, kw["block_step"])[0]) * len(ref_positions(w, h, kw["block_step"])[1])
-    per_kernel = {k: (ms / max(cnt, 1), cnt) for k, (ms, cnt) in times.items()}
-    dominant = max(times, key=lambda k: times[k][0])
-    avg_ms = per_kernel[dominant][0]
-    tw = 2 * kw.get("radius", 0) + 1
-    if dominant == "bm3d_match":
-        work = matching_flops(w, h, kw["block_step"], kw["bm_range"])
-        achieved = work / (avg_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None, kernel=dominant,
-                    work_per_launch=work, work_unit="FLOP (spatial matching, reference count)")
-    elif dominant == "bm3d_estimate":
-        work = n_ref * TRANSFORM_FLOP_PER_GROUP
-        achieved = work / (avg_ms * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None, kernel=dominant,
-                    work_per_launch=work, work_unit="FLOP (3-D transforms + threshold)")
-    else:
-        work = (2 * tw + 1) * w * h * 4.0
-        achieved = work / (avg_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
-                    frac=achieved / PEAK_HBM_GBS, traffic=None, kernel=dominant,
-                    work_per_launch=work, work_unit="bytes")
+    extra = {}
+    if "fwd_bwd" in forms:
+        c5, kw5, s5, pk5 = bench_fwd_bwd(args.steps, args.warmup)
+        extra["fwd_bwd"] = dict(value=c5.shape[0] * args.steps / s5, unit="frames/s",
+                                ms_per_step=s5 * 1e3 / args.steps,
+                                workload=describe(5, c5.shape, kw5, "forward with saved state + backward"),
+                                kernels_ms_avg={k: v[0] for k, v in pk5.items()})
+    if "temporal_r2" in forms:
+        c3, kw3, s3, pk3 = bench_plain(3, max(1, args.steps // 2), 1, args.frames)
+        extra["temporal_r2"] = dict(value=c3.shape[0] * max(1, args.steps // 2) / s3, unit="frames/s",
+                                    workload=describe(3, c3.shape, kw3, "plain form, luma"),
+                                    kernels_ms_avg={k: v[0] for k, v in pk3.items()})
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -152,20 +248,17 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": secs * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"config {args.config}: {nf} x {w}x{h} GrayS float32, plain form, "
-                        + ", ".join(f"{k}={v}" for k, v in kw.items()),
-            "frames_per_step": nf,
-        },
+        "config": {"workload": describe(args.config, clip.shape, kw, "plain form"), "frames_per_step": nf},
         "kernels_ms_avg": {k: v[0] for k, v in per_kernel.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "forms": extra,
     }
     print(json.dumps(line), flush=True)
 
