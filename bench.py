@@ -4,24 +4,28 @@
 A step is one pass of the filter over the whole synthetic clip resident in
 HBM: for the default workload, 32 frames of 1920x1080 GrayS float32 with
 sigma 3, block_step 4, bm_range 16, radius 0 (plain FP32 form).  `value` is
-frames per second over the timed steps.  Per-kernel durations come from HIP
-events recorded around every launch on the launch stream inside the timed
-region; the dominant kernel's algorithmic work divided by its average
-duration gives the roofline fraction.  The CPU baseline is the C restatement
-of the reference shaders (the oracle) timed on a bounded sample of the same
-workload on the host cores.
+frames per second over the timed steps (each step's launch sequence replayed
+from the library's HIP graph).  Per-kernel durations come from HIP events
+the library records around every launch on the launch stream, over further
+steps launched directly; the dominant kernel's algorithmic work divided by
+its average duration gives the roofline fraction.  The CPU baseline is the
+C restatement of the reference shaders (the oracle) timed on a bounded
+sample of the same workload on the host cores.
 
 Beside the headline line's `value`, the same JSON line carries the other
-forms, each timed the same way (`forms`):
+forms (`forms`):
 
 * ``fwd_bwd``: the differentiable form, 16 frames of 512x512 (sigma 3,
   block_step 4, bm_range 9, radius 0): forward with saved groups, masks and
   weights, then the backward pass, frames/s of forward+backward;
 * ``temporal_r2``: 32 frames of 1920x1080 luma shifted by (1.5t, 0.75t) px,
-  radius 2, ps_num 2, ps_range 7, extractor_exp 2.
+  radius 2, ps_num 2, ps_range 7, extractor_exp 2;
+* ``host_io``: the headline workload through the host-buffer entry point
+  (the VapourSynth getFrame path): upload, filter and download per frame
+  request, so PCIe-inclusive.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
-                    [--forms plain,fwd_bwd,temporal_r2] [--cpu-seconds S]
+                    [--forms plain,fwd_bwd,temporal_r2,host_io] [--cpu-seconds S]
                     [--copy-calibration-mb MB]
 """
 
@@ -96,19 +100,23 @@ def copy_calibration(mb):
 
 
 def timed(inst, run, steps, warmup):
-    """Warm up, then time `steps` calls of run() with the library's per-kernel
-    HIP events on; returns (seconds, {kernel: (avg ms, launches)})."""
+    """Warm up, time `steps` calls of run() (launch sequences replayed from
+    HIP graphs where the entry point has them), then run max(2, steps // 2)
+    more calls with the library's per-kernel HIP events on (direct launches)
+    for the kernel durations; returns (seconds, {kernel: (avg ms, launches)})."""
     import torch
     for _ in range(warmup):
         run()
-    torch.cuda.synchronize()
-    inst.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         run()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    inst.set_timing(True)
+    for _ in range(max(2, steps // 2)):
+        run()
+    torch.cuda.synchronize()
     times = inst.kernel_times()
     inst.set_timing(False)
     per_kernel = {k: (ms / cnt, cnt) for k, (ms, cnt) in times.items() if cnt > 0}
@@ -190,7 +198,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--frames", type=int, default=None, help="override the clip length")
-    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2",
+    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2,host_io",
                     help="comma list of forms to time (plain is the headline value)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-time budget of the CPU baseline sample (0: skip)")
@@ -220,6 +228,19 @@ def main():
                                 ms_per_step=s5 * 1e3 / args.steps,
                                 workload=describe(5, c5.shape, kw5, "forward with saved state + backward"),
                                 kernels_ms_avg={k: v[0] for k, v in pk5.items()})
+    if "host_io" in forms:
+        from vsfeel import BM3Dv2, Clip
+        node = BM3Dv2(Clip.gray(clip), **kw)
+        node.get_frame(0)
+        h0 = time.perf_counter()
+        for n in range(nf):
+            node.get_frame(n)
+        h1 = time.perf_counter()
+        extra["host_io"] = dict(value=nf / (h1 - h0), unit="frames/s",
+                                workload=describe(args.config, clip.shape, kw,
+                                                  "host buffers: upload, filter and download per "
+                                                  "frame request, one thread (PCIe-inclusive)"))
+        del node
     if "temporal_r2" in forms:
         c3, kw3, s3, pk3 = bench_plain(3, max(1, args.steps // 2), 1, args.frames)
         extra["temporal_r2"] = dict(value=c3.shape[0] * max(1, args.steps // 2) / s3, unit="frames/s",

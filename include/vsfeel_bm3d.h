@@ -216,6 +216,17 @@ int vsfeel_bm3d_backward(vsfeel_bm3d *h, const float *grad_out, int32_t grad_pit
 int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
 int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);
 
+/*
+ * Graph replay of vsfeel_bm3d_process_device (on by default): a call
+ * captures its launch sequence (per output frame: slot clears, matching,
+ * filtering, tile reduction, aggregation) into a HIP graph on the instance's
+ * stream, and later calls with identical arguments replay it, ordered after
+ * and before the caller's stream by events.  The counterpart of the
+ * reference's per-frame command-buffer recording (src/bm3d.cpp:486-623).
+ * Timing (vsfeel_bm3d_set_timing) launches directly instead.
+ */
+int vsfeel_bm3d_set_graphs(vsfeel_bm3d *h, int32_t enable);
+
 #ifdef __cplusplus
 }
 #endif

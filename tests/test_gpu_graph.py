@@ -1,0 +1,73 @@
+"""Graph replay of the device entry point against direct launches.
+
+process() captures its launch sequence into a HIP graph on the first call
+with a given set of arguments and replays it afterwards.  Every replay, with
+the program's own eager work and synchronisations between replays, must give
+bit-identical frames to the directly launched sequence (the accumulation is
+integer fixed point, so the results are deterministic), also after the clip
+memory is rewritten in place and after the output moves.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "spatial": (lambda s: s.workload(1, frames=3),
+                dict(sigma=3.0, block_step=8, bm_range=9, radius=0)),
+    "temporal": (lambda s: s.workload(3, frames=5, width=192, height=112)[0],
+                 dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7,
+                      extractor_exp=2)),
+}
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_graph_replays_match_direct_launches(name):
+    torch = _torch()
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+
+    make, kw = CASES[name]
+    clip = np.ascontiguousarray(make(synth), np.float32)
+    nf, h, w = clip.shape
+    print(f"graph case {name} clip={clip.shape} args={kw}", flush=True)
+    x = torch.from_numpy(clip).to("cuda:0")
+    direct = Instance(nf, h, w, **kw)
+    direct.set_graphs(False)
+    want = direct.process(x).cpu().numpy()
+
+    inst = Instance(nf, h, w, **kw)
+    out = torch.empty_like(x)
+    for rep in range(3):
+        got = inst.process(x, out=out)
+        scratch = (x * 2.0 + 1.0).sum()  # eager work of the program between replays
+        torch.cuda.synchronize()
+        assert np.isfinite(float(scratch))
+        assert np.array_equal(got.cpu().numpy(), want), f"{name}: replay {rep} differs"
+
+    # same memory, new content: the replay reads the clip at run time
+    clip2 = np.ascontiguousarray(clip[::-1])
+    x.copy_(torch.from_numpy(clip2))
+    want2 = direct.process(x).cpu().numpy()
+    got2 = inst.process(x, out=out).cpu().numpy()
+    assert np.array_equal(got2, want2), f"{name}: replay after rewriting the clip differs"
+
+    # a new output buffer is a new capture
+    out2 = torch.empty_like(x)
+    got3 = inst.process(x, out=out2).cpu().numpy()
+    assert np.array_equal(got3, want2), f"{name}: capture for a new output differs"
+
+    # a sub-range of frames
+    got4 = inst.process(x, first=1, count=nf - 2).cpu().numpy()
+    want4 = direct.process(x, first=1, count=nf - 2).cpu().numpy()
+    assert np.array_equal(got4, want4), f"{name}: sub-range replay differs"

@@ -55,6 +55,18 @@ CASES = {
     # flat content: every distance ties, the seq order decides
     "flat": (lambda s: np.full((2, 40, 48), 0.5, np.float32),
              dict(sigma=3.0, block_step=4, bm_range=7, radius=0)),
+    # ... also in the temporal search (overlapping predictor windows, every
+    # frame top-8 decided by seq, more than 16 tied candidates)
+    "flat_temporal": (lambda s: np.full((5, 40, 48), 0.25, np.float32),
+                      dict(sigma=3.0, block_step=4, bm_range=5, radius=2, ps_num=3, ps_range=3)),
+    # wide predictor windows (25 x 25, three per frame): one ref block per
+    # workgroup of the temporal search
+    "wide_predictive": (lambda s: s.shifted_clip(12, 5, 120, 96, 3 / 255, (2.0, -1.0)),
+                        dict(sigma=3.0, block_step=6, bm_range=8, radius=2, ps_num=3, ps_range=12)),
+    # predictor windows too wide for the row-parallel temporal search
+    # (61 columns + 7): the per-wavefront search takes over
+    "widest_predictive": (lambda s: s.shifted_clip(13, 3, 96, 80, 3 / 255, (1.0, 1.0)),
+                          dict(sigma=3.0, block_step=8, bm_range=6, radius=1, ps_num=2, ps_range=30)),
 }
 
 
@@ -113,6 +125,27 @@ def test_fp_modes_match_oracle(name, fp_mode):
     got, ref, _ = _run_case(name, fp_mode)
     err = float(np.abs(got - ref).max())
     assert err <= TOL, f"{name} fp_mode {fp_mode}: max |gpu - oracle| = {err}"
+
+
+@pytest.mark.parametrize("name", ["spatial_96x80", "temporal_72x56_r2", "exact_48x40_r3"])
+def test_matches_golden_vectors(name):
+    """The committed fixtures (tests/golden, written by the oracle)."""
+    torch = _torch()
+    from test_oracle import load_golden
+    from vsfeel.torch_bm3d import Instance
+
+    clip, kw, groups, want = load_golden(name)
+    nf, h, w = clip.shape
+    print(f"golden {name} clip={clip.shape} args={kw}", flush=True)
+    inst = Instance(nf, h, w, **kw)
+    x = torch.from_numpy(np.ascontiguousarray(clip)).to("cuda:0")
+    got = inst.process(x).cpu().numpy()
+    for m in range(nf):
+        g, _ = inst.groups(x, m)
+        assert np.array_equal(g, groups[m]), f"{name}: frame {m} groups differ from the fixture"
+    if kw.get("extractor_exp", 0):
+        assert np.array_equal(got, want), f"{name}: extractor sums must be exact"
+    assert float(np.abs(got - want).max()) <= TOL
 
 
 def test_final_pass_matches_oracle():

@@ -11,6 +11,9 @@ constant frame stays constant, the extractor quantises to its grid).
 
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 import pytest
 from scipy.fft import dct, dctn, idct, idctn
@@ -256,3 +259,34 @@ def test_f64_oracle_close_to_f32():
     a = po.clip(p, clip)
     b = po.clip(p, clip, f64=True)
     assert np.abs(a - b).max() < 5e-3
+
+
+# ------------------------------------------------------------ golden vectors
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_golden(name):
+    """(clip, BM3Dv2 arguments, groups (nf, ny, nx, 8, 3), denoised frames)."""
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    kw = json.loads(bytes(z["args"]).decode())
+    return z["clip"], kw, z["groups"].astype(np.int32), z["out"]
+
+
+GOLDEN_CASES = ["spatial_96x80", "temporal_72x56_r2", "exact_48x40_r3"]
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_oracle_reproduces_the_golden_vectors(name):
+    clip, kw, groups, out = load_golden(name)
+    nf, h, w = clip.shape
+    p = po.make_params(w, h, nthreads=3, **kw)
+    got = po.clip(p, clip)
+    r = kw.get("radius", 0)
+    for m in range(nf):
+        window = [clip[min(max(m - r + z, 0), nf - 1)] for z in range(2 * r + 1)]
+        _, g = po.estimate(p, window)
+        assert np.array_equal(g, groups[m]), f"{name}: frame {m} groups changed"
+    if kw.get("extractor_exp", 0):
+        assert np.array_equal(got, out)
+    else:
+        np.testing.assert_allclose(got, out, rtol=0, atol=1e-6)

@@ -128,6 +128,9 @@ struct Instance {
     // workgroup run and the LDS layout; run_blocks == 0 selects the
     // per-wavefront search instead
     int run_blocks = 0, runs_x = 0, region_cols = 0, region_rows = 0, dist_cols = 0;
+    // row-parallel temporal search (bm3d_match_temporal): ref blocks per
+    // workgroup; 0 selects the per-wavefront search
+    int t_blocks = 0;
     float *dst_dev = nullptr;    // one plane
 
     // tiled accumulation of the frame's own slice (see bm3d_estimate_tiled)
@@ -143,6 +146,25 @@ struct Instance {
     std::vector<int> src_frame;  // frame held by each src ring slot (-1: none)
     std::vector<int> res_frame;  // frame whose estimate stack each res slot holds
     std::mutex mu;
+
+    // process_device replays its launch sequence from a HIP graph while the
+    // call's arguments repeat (captured on the first call with new ones)
+    struct GraphKey {
+        const float *data, *ref;
+        int64_t frame_stride, out_frame_stride;
+        int32_t pitch, first, count, out_pitch;
+        float *out;
+        bool operator==(const GraphKey &o) const
+        {
+            return data == o.data && ref == o.ref && frame_stride == o.frame_stride &&
+                   out_frame_stride == o.out_frame_stride && pitch == o.pitch && first == o.first &&
+                   count == o.count && out_pitch == o.out_pitch && out == o.out;
+        }
+    };
+    bool graphs = true;
+    hipGraphExec_t graph_exec = nullptr;
+    GraphKey graph_key{};
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;  // orders the instance stream with a null-stream caller
 
     // optional per-kernel timing: HIP events recorded around every launch on
     // the launch stream, harvested on request
@@ -239,6 +261,15 @@ struct Instance {
         if (last_done) {
             (void)hipEventDestroy(last_done);
         }
+        if (graph_exec) {
+            (void)hipGraphExecDestroy(graph_exec);
+        }
+        if (fork_ev) {
+            (void)hipEventDestroy(fork_ev);
+        }
+        if (join_ev) {
+            (void)hipEventDestroy(join_ev);
+        }
         (void)harvest();
         for (hipEvent_t e : event_pool) {
             (void)hipEventDestroy(e);
@@ -279,6 +310,7 @@ struct Instance {
         m.region_cols = region_cols;
         m.region_rows = region_rows;
         m.dist_cols = dist_cols;
+        m.t_blocks = t_blocks;
         return m;
     }
 
@@ -585,6 +617,19 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
                 d.dist_cols = dcols;
             }
         }
+        if (d.run_blocks > 0 && d.radius > 0) {
+            // temporal search: one thread per (block, direction, predictor
+            // window, window row); a staged window row of at most 64 columns
+            const int wh = 2 * d.ps_range + 1;
+            const int rpb = 2 * d.ps_num * wh;
+            if (wh + 7 <= 64 && rpb <= 256) {
+                int nbt = 256 / rpb;
+                while (nbt > 0 && temporal_lds_bytes(nbt, d.ps_num, d.ps_range) > 64 * 1024) {
+                    --nbt;
+                }
+                d.t_blocks = nbt;
+            }
+        }
     }
     d.tiled = choose_tiles(d.width, d.height, d.block_step, d.bm_range, &d.tile_bx, &d.tile_by, &d.region_w,
                            &d.region_h);
@@ -643,7 +688,9 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
     if ((e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking)) != hipSuccess) {
         return dev_fail(e, "hipStreamCreate");
     }
-    if ((e = hipEventCreateWithFlags(&d.last_done, hipEventDisableTiming)) != hipSuccess) {
+    if ((e = hipEventCreateWithFlags(&d.last_done, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&d.fork_ev, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&d.join_ev, hipEventDisableTiming)) != hipSuccess) {
         return dev_fail(e, "hipEventCreate");
     }
     const size_t plane = size_t(d.pe) * sizeof(float);
@@ -872,26 +919,74 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
     hipStream_t s = static_cast<hipStream_t>(stream);
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
     const int r = d.radius, nf = d.nframes;
-    // estimate stacks are only reused inside one call: the caller owns the
-    // clip memory and may rewrite it between calls
-    std::vector<int> held(d.res_cap, -1);
-    std::vector<int> slot_of(d.tw);
-    for (int n = first; n < first + count; ++n) {
-        for (int i = 0; i < d.tw; ++i) {
-            const int m = clampi(n - r + i, 0, nf - 1);
-            const int slot = m % d.res_cap;
-            slot_of[i] = slot;
-            if (held[slot] == m) {
-                continue;
+    // the whole call's launch sequence; estimate stacks are only reused
+    // inside one call: the caller owns the clip memory and may rewrite it
+    // between calls
+    auto enqueue = [&](hipStream_t q) -> hipError_t {
+        std::vector<int> held(d.res_cap, -1);
+        std::vector<int> slot_of(d.tw);
+        for (int n = first; n < first + count; ++n) {
+            for (int i = 0; i < d.tw; ++i) {
+                const int m = clampi(n - r + i, 0, nf - 1);
+                const int slot = m % d.res_cap;
+                slot_of[i] = slot;
+                if (held[slot] == m) {
+                    continue;
+                }
+                const float *search[kMaxWindow];
+                const float *input[kMaxWindow];
+                device_window(d, clip, m, search, input);
+                hipError_t e = d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, q);
+                if (e != hipSuccess) {
+                    return e;
+                }
+                held[slot] = m;
             }
-            const float *search[kMaxWindow];
-            const float *input[kMaxWindow];
-            device_window(d, clip, m, search, input);
-            VSFEEL_HIP(d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, s), "estimate launch");
-            held[slot] = m;
+            hipError_t e = d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, q);
+            if (e != hipSuccess) {
+                return e;
+            }
         }
-        VSFEEL_HIP(d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, s),
-                   "aggregate launch");
+        return hipSuccess;
+    };
+    if (d.graphs && !d.timing && count > 0) {
+        // replay from a graph captured on the instance stream, ordered after
+        // and before the caller's stream by events
+        const Instance::GraphKey key{clip->data, clip->ref, clip->frame_stride, out_frame_stride,
+                                     clip->pitch, first, count, out_pitch, out};
+        if (!d.graph_exec || !(d.graph_key == key)) {
+            if (d.graph_exec) {
+                // the previous graph may still run on the instance stream
+                VSFEEL_HIP(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+                (void)hipGraphExecDestroy(d.graph_exec);
+                d.graph_exec = nullptr;
+            }
+            VSFEEL_HIP(hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            const hipError_t e = enqueue(d.stream);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(d.stream, &g);
+            if (e != hipSuccess || ec != hipSuccess) {
+                if (g) {
+                    (void)hipGraphDestroy(g);
+                }
+                VSFEEL_HIP(e, "launch during graph capture");
+                VSFEEL_HIP(ec, "hipStreamEndCapture");
+            }
+            const hipError_t ei = hipGraphInstantiate(&d.graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                d.graph_exec = nullptr;
+                VSFEEL_HIP(ei, "hipGraphInstantiate");
+            }
+            d.graph_key = key;
+        }
+        VSFEEL_HIP(hipEventRecord(d.fork_ev, s), "hipEventRecord");
+        VSFEEL_HIP(hipStreamWaitEvent(d.stream, d.fork_ev, 0), "hipStreamWaitEvent");
+        VSFEEL_HIP(hipGraphLaunch(d.graph_exec, d.stream), "hipGraphLaunch");
+        VSFEEL_HIP(hipEventRecord(d.join_ev, d.stream), "hipEventRecord");
+        VSFEEL_HIP(hipStreamWaitEvent(s, d.join_ev, 0), "hipStreamWaitEvent");
+    } else {
+        VSFEEL_HIP(enqueue(s), "launch");
     }
     // the res slots now hold this call's stacks, not the host path's frames
     std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
@@ -1090,6 +1185,27 @@ int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable)
         d.kernel_count[k] = 0;
     }
     d.timing = enable != 0;
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_set_graphs(vsfeel_bm3d *h, int32_t enable)
+{
+    if (!h) {
+        return VSFEEL_BM3D_EINVAL;
+    }
+    Instance &d = h->inst;
+    std::lock_guard<std::mutex> lock(d.mu);
+    d.graphs = enable != 0;
+    if (!d.graphs && d.graph_exec) {
+        if (d.device >= 0) {
+            (void)hipSetDevice(d.device);
+        }
+        if (d.stream) {
+            (void)hipStreamSynchronize(d.stream);
+        }
+        (void)hipGraphExecDestroy(d.graph_exec);
+        d.graph_exec = nullptr;
+    }
     return VSFEEL_BM3D_OK;
 }
 

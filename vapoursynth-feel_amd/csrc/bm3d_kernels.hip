@@ -500,6 +500,315 @@ __device__ __forceinline__ void sort16_dpp(uint64_t &key, uint32_t &xy, int lane
 }
 
 // ---------------------------------------------------------------------------
+// One window row of the sliding metric from a column-major LDS region (column
+// j of the row starts at col0 + j * CS): column SSDs v_j against reference
+// column min(j, 7) of cb, then
+// D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
+// written to drow[0 .. rw-1].  Returns the row's 3 smallest distances
+// (x <= y <= z; distances at or above FLT_MAX, and NaN, count as FLT_MAX)
+// and in w the positions of the first two (k1 | k2 << 16).
+// ---------------------------------------------------------------------------
+template <int CM>
+__device__ __forceinline__ float4 scan_row_sliding(const float *cb, const float *col0, int CS, int rw, float *drow)
+{
+    float c7[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        c7[i] = cb[56 + i];
+    }
+    float m1 = FLT_MAX, m2 = FLT_MAX, m3 = FLT_MAX;
+    uint32_t k1 = 0, k2 = 0;
+    float v[15];
+    for (int k0 = 0; k0 < rw; k0 += 8) {
+        if (k0 == 0) {
+#pragma unroll
+            for (int j = 0; j < 15; ++j) {
+                if (j <= rw + 6) {
+                    const float *sp = col0 + j * CS;
+                    float sv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        sv[i] = sp[i];
+                    }
+                    if (j < 7) {
+                        float cc[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            cc[i] = cb[j * 8 + i];
+                        }
+                        v[j] = col_ssd<CM>(cc, sv);
+                    } else {
+                        v[j] = col_ssd<CM>(c7, sv);
+                    }
+                } else {
+                    v[j] = 0.0f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                v[j] = v[j + 8];
+            }
+#pragma unroll
+            for (int j = 7; j < 15; ++j) {
+                if (k0 + j <= rw + 6) {
+                    const float *sp = col0 + (k0 + j) * CS;
+                    float sv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        sv[i] = sp[i];
+                    }
+                    v[j] = col_ssd<CM>(c7, sv);
+                } else {
+                    v[j] = 0.0f;
+                }
+            }
+        }
+        float p1[14], p2[12];
+#pragma unroll
+        for (int j = 0; j < 14; ++j) {
+            p1[j] = v[j] + v[j + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+            p2[j] = p1[j] + p1[j + 2];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (k0 + i < rw) {
+                const float d = p2[i] + p2[i + 4];
+                drow[k0 + i] = d;
+                // FLT_MAX and NaN never count (the shader's strict '<')
+                const float dv = (d < FLT_MAX) ? d : FLT_MAX;
+                const bool c1 = dv < m1, c2 = dv < m2;
+                k2 = c1 ? k1 : (c2 ? uint32_t(k0 + i) : k2);
+                k1 = c1 ? uint32_t(k0 + i) : k1;
+                const float t1 = fmaxf(m1, dv);
+                m1 = fminf(m1, dv);
+                const float t2 = fmaxf(m2, t1);
+                m2 = fminf(m2, t1);
+                m3 = fminf(m3, t2);
+            }
+        }
+    }
+    return make_float4(m1, m2, m3, __uint_as_float(k1 | (k2 << 16)));
+}
+
+// Row geometry of one search for the selection: row length, candidate seq
+// and position.  A spatial window: rows top.., one length, seq = row*rw + k.
+struct SpatialRows {
+    int rw, left, top;
+    __device__ int len(int) const { return rw; }
+    __device__ int seq(int row, int k) const { return row * rw + k; }
+    __device__ uint32_t xy(int row, int k) const { return uint32_t(left + k) | (uint32_t(top + row) << 16); }
+};
+
+// The ps_num predictor windows of one temporal step: row slot rho = w*wh + r
+// (rows r >= the window's height are empty), seq = w*win + r*rw_w + k
+// (src/bm3d.comp:569-590).
+struct PredictiveRows {
+    const int4 *geo;  // per window: left, top, rw, rh
+    int wh, win;
+    __device__ int len(int rho) const { return geo[rho / wh].z; }
+    __device__ int seq(int rho, int k) const
+    {
+        const int w = rho / wh;
+        return w * win + (rho - w * wh) * geo[w].z + k;
+    }
+    __device__ uint32_t xy(int rho, int k) const
+    {
+        const int w = rho / wh;
+        const int4 g = geo[w];
+        return uint32_t(g.x + k) | (uint32_t(g.y + rho - w * wh) << 16);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Top-8 (distance, seq) keys of one search by a 16-lane row of the wavefront
+// (sub = lane & 15), from the 3 smallest distances of each of its nrows rows
+// (rt) and its distance rows (db, row stride dcols).  The threshold is the
+// 8th smallest of the stored distances, ties counted: they are distances of
+// the search, so it bounds the search's 8th smallest from above.  Each lane
+// keeps the sorted 8 smallest of its rows (sub, sub+16, ...) and the row
+// takes the 8 smallest heads in turn.  The candidates at or below it are
+// gathered into the row's 16-entry stage: a row whose third smallest is above
+// it holds them among its stored first two, the other rows (usually the
+// best match's row and a neighbour) are scanned.  Returns their number and
+// sets thr; when it is at most 16, lanes 0..7 of the row hold the sorted
+// top-8 in key/kxy (placeholders where the search has fewer than 8
+// candidates).  nrows is the same for all rows of the wavefront.
+// ---------------------------------------------------------------------------
+template <class Geo>
+__device__ int select16(const float4 *rt, const float *db, int dcols, int nrows, const Geo &geo, bool act, int lane,
+                        unsigned long long *sk, uint32_t *sxy, uint64_t &key, uint32_t &kxy, float &thr)
+{
+    const int grp = lane >> 4, sub = lane & 15;
+    const int nslots = (nrows + 15) >> 4;
+    float h[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        h[i] = FLT_MAX;
+    }
+    for (int r = sub; r < nrows; r += 16) {
+        const float4 q = rt[r];
+        const float qv[3] = {q.x, q.y, q.z};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float v = qv[j];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float lo = fminf(h[i], v);
+                v = fmaxf(h[i], v);
+                h[i] = lo;
+            }
+        }
+    }
+    thr = FLT_MAX;
+    int taken = 0;
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const float m = row_min16(h[0], lane);
+        const bool hit = h[0] == m;
+        const uint32_t seg = uint32_t(__ballot(hit) >> (grp * 16)) & 0xFFFFu;
+        if (taken < 8) {
+            taken += __popc(seg);
+            thr = m;
+        }
+        if (hit) {
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                h[i] = h[i + 1];
+            }
+            h[7] = FLT_MAX;
+        }
+    }
+    // stored survivors (2 bits per row slot) and rows to scan (1 bit)
+    uint32_t own = 0, scan = 0;
+    int cnt = 0;
+    for (int q = 0; q < nslots; ++q) {
+        const int r = q * 16 + sub;
+        if (act && r < nrows) {
+            const float4 v = rt[r];
+            if (v.z <= thr && v.z < FLT_MAX) {
+                scan |= 1u << q;
+            } else {
+                const uint32_t s0 = (v.x <= thr && v.x < FLT_MAX) ? 1u : 0u;
+                const uint32_t s1 = (v.y <= thr && v.y < FLT_MAX) ? 1u : 0u;
+                own |= (s0 | (s1 << 1)) << (2 * q);
+                cnt += int(s0 + s1);
+            }
+        }
+    }
+    const int incl = row_prefix_sum16(cnt);
+    int pos = incl - cnt;
+    const int stored = __shfl(incl, (lane & ~15) | 15);
+    for (int q = 0; own != 0; ++q, own >>= 2) {
+        if ((own & 3u) == 0) {
+            continue;
+        }
+        const int r = q * 16 + sub;
+        const float4 v = rt[r];
+        const uint32_t kk = __float_as_uint(v.w);
+        if ((own & 1u) && pos < 16) {
+            const int k = int(kk & 0xFFFFu);
+            sk[pos] = make_key(v.x, geo.seq(r, k));
+            sxy[pos] = geo.xy(r, k);
+        }
+        pos += int(own & 1u);
+        if ((own & 2u) && pos < 16) {
+            const int k = int(kk >> 16);
+            sk[pos] = make_key(v.y, geo.seq(r, k));
+            sxy[pos] = geo.xy(r, k);
+        }
+        pos += int((own >> 1) & 1u);
+    }
+    int staged = stored;
+    for (int q = 0; q < nslots; ++q) {
+        uint32_t seg = uint32_t(__ballot((scan >> q) & 1u) >> (grp * 16)) & 0xFFFFu;
+        while (seg) {
+            const int r = q * 16 + __ffs(seg) - 1;
+            seg &= seg - 1;
+            const int rw = geo.len(r);
+            for (int k0 = 0; k0 < rw; k0 += 16) {
+                const int k = k0 + sub;
+                const float d = k < rw ? db[r * dcols + k] : FLT_MAX;
+                const bool surv = k < rw && d <= thr && d < FLT_MAX;
+                const uint32_t sm = uint32_t(__ballot(surv) >> (grp * 16)) & 0xFFFFu;
+                const int p = staged + __popc(sm & ((1u << sub) - 1u));
+                if (surv && p < 16) {
+                    sk[p] = make_key(d, geo.seq(r, k));
+                    sxy[p] = geo.xy(r, k);
+                }
+                staged += __popc(sm);
+            }
+        }
+    }
+    wave_lds_sync();
+    key = sub < staged && sub < 16 ? sk[sub] : kNoKey;
+    kxy = sub < staged && sub < 16 ? sxy[sub] : 0u;
+    wave_lds_sync();
+    if (staged <= 16) {
+        sort16_dpp(key, kxy, lane);
+        if (!(key < kPlaceholderKey)) {
+            key = kPlaceholderKey;  // fewer than 8 candidates in the search
+            kxy = 0u;
+        }
+    }
+    return staged;
+}
+
+// The same search by a whole wavefront, for more than 16 candidates at or
+// below thr (ties): every row holding one is scanned, the candidates staged
+// in batches of 64 and merged into the list.
+template <class Geo>
+__device__ void select_wave(const float4 *rt, const float *db, int dcols, int nrows, const Geo &geo, float thr,
+                            int lane, unsigned long long *sk, uint32_t *sxy, TopList &L)
+{
+    list_reset(L);
+    int staged = 0;
+    for (int r0 = 0; r0 < nrows; r0 += 64) {
+        bool any = false;
+        if (r0 + lane < nrows) {
+            const float m = rt[r0 + lane].x;
+            any = m <= thr && m < FLT_MAX;
+        }
+        uint64_t rows = __ballot(any);
+        while (rows) {
+            const int r = r0 + __ffsll((unsigned long long)rows) - 1;
+            rows &= rows - 1;
+            const int rw = geo.len(r);
+            for (int k0 = 0; k0 < rw; k0 += 64) {
+                const int k = k0 + lane;
+                const float d = k < rw ? db[r * dcols + k] : FLT_MAX;
+                const bool surv = k < rw && d <= thr && d < FLT_MAX;
+                const uint64_t m = __ballot(surv);
+                const int cnt = __popcll(m);
+                if (staged + cnt > 64) {
+                    wave_lds_sync();
+                    const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+                    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+                    wave_lds_sync();
+                    list_merge_batch(L, key, kxy, lane);
+                    staged = 0;
+                }
+                if (surv) {
+                    const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
+                    sk[p] = make_key(d, geo.seq(r, k));
+                    sxy[p] = geo.xy(r, k);
+                }
+                staged += cnt;
+            }
+        }
+    }
+    wave_lds_sync();
+    const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+    wave_lds_sync();
+    list_merge_batch(L, key, kxy, lane);
+}
+
+// ---------------------------------------------------------------------------
 // Spatial search of the sliding metric, row-parallel.  A workgroup takes a
 // run of run_blocks consecutive ref blocks of one grid row; their windows
 // share the rows [top, bottom], so one source region (column-major in LDS)
@@ -529,6 +838,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
     float *cur = region + A.region_cols * CS;                               // [NB][64]
     float *dist = cur + NB * 64;                                            // [NB][rh][dist_cols]
     int *stage_over = reinterpret_cast<int *>(dist + NB * RH * A.dist_cols);  // [NB] more than 16 survivors
+    float *stage_thr = reinterpret_cast<float *>(stage_over + NB);             // [NB] their threshold
 
     const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
     const int run = xcd_block_id();
@@ -563,91 +873,8 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int b = t / rh, r = t - b * rh;
         const int xb = min((bx0 + b) * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
-        const int rw = right - left + 1;
-        const float *cb = cur + b * 64;
-        float c7[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            c7[i] = cb[56 + i];
-        }
-        const float *col0 = region + (left - rx0) * CS + r;  // column j at col0 + j*CS
-        float *drow = dist + (b * rh + r) * A.dist_cols;
-        float m1 = FLT_MAX, m2 = FLT_MAX, m3 = FLT_MAX;  // the row's 3 smallest distances
-        uint32_t k1 = 0, k2 = 0;                          // positions of m1, m2
-        float v[15];
-        for (int k0 = 0; k0 < rw; k0 += 8) {
-            if (k0 == 0) {
-#pragma unroll
-                for (int j = 0; j < 15; ++j) {
-                    if (j <= rw + 6) {
-                        const float *sp = col0 + j * CS;
-                        float sv[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            sv[i] = sp[i];
-                        }
-                        if (j < 7) {
-                            float cc[8];
-#pragma unroll
-                            for (int i = 0; i < 8; ++i) {
-                                cc[i] = cb[j * 8 + i];
-                            }
-                            v[j] = col_ssd<CM>(cc, sv);
-                        } else {
-                            v[j] = col_ssd<CM>(c7, sv);
-                        }
-                    } else {
-                        v[j] = 0.0f;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 7; ++j) {
-                    v[j] = v[j + 8];
-                }
-#pragma unroll
-                for (int j = 7; j < 15; ++j) {
-                    if (k0 + j <= rw + 6) {
-                        const float *sp = col0 + (k0 + j) * CS;
-                        float sv[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            sv[i] = sp[i];
-                        }
-                        v[j] = col_ssd<CM>(c7, sv);
-                    } else {
-                        v[j] = 0.0f;
-                    }
-                }
-            }
-            float p1[14], p2[12];
-#pragma unroll
-            for (int j = 0; j < 14; ++j) {
-                p1[j] = v[j] + v[j + 1];
-            }
-#pragma unroll
-            for (int j = 0; j < 12; ++j) {
-                p2[j] = p1[j] + p1[j + 2];
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if (k0 + i < rw) {
-                    const float d = p2[i] + p2[i + 4];
-                    drow[k0 + i] = d;
-                    // FLT_MAX and NaN never count (the shader's strict '<')
-                    const float dv = (d < FLT_MAX) ? d : FLT_MAX;
-                    const bool c1 = dv < m1, c2 = dv < m2;
-                    k2 = c1 ? k1 : (c2 ? uint32_t(k0 + i) : k2);
-                    k1 = c1 ? uint32_t(k0 + i) : k1;
-                    const float t1 = fmaxf(m1, dv);
-                    m1 = fminf(m1, dv);
-                    const float t2 = fmaxf(m2, t1);
-                    m2 = fminf(m2, t1);
-                    m3 = fminf(m3, t2);
-                }
-            }
-        }
-        rowtop[b * rh + r] = make_float4(m1, m2, m3, __uint_as_float(k1 | (k2 << 16)));
+        rowtop[b * rh + r] = scan_row_sliding<CM>(cur + b * 64, region + (left - rx0) * CS + r, CS,
+                                                  right - left + 1, dist + (b * rh + r) * A.dist_cols);
     }
     __syncthreads();
 
@@ -663,126 +890,13 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int bx = bx0 + bs;
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
-        const int rw = right - left + 1;
-        const float *db = dist + bs * rh * A.dist_cols;
-        const float4 *rt = rowtop + bs * rh;
-        // threshold: the 8th smallest (ties counted) of the rows' stored 3
-        // smallest distances.  They are distances of the window, so it bounds
-        // the window's 8th smallest distance from above.  Each lane keeps the
-        // sorted 8 smallest of its rows (sub, sub+16, ...) and the row takes
-        // the 8 smallest heads in turn.
-        float h[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            h[i] = FLT_MAX;
-        }
-        for (int r = sub; r < rh; r += 16) {
-            const float4 q = rt[r];
-            const float qv[3] = {q.x, q.y, q.z};
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                float v = qv[j];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const float lo = fminf(h[i], v);
-                    v = fmaxf(h[i], v);
-                    h[i] = lo;
-                }
-            }
-        }
-        float thr = FLT_MAX;
-        int taken = 0;
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const float m = row_min16(h[0], lane);
-            const bool hit = h[0] == m;
-            const uint32_t seg = uint32_t(__ballot(hit) >> (grp * 16)) & 0xFFFFu;
-            if (taken < 8) {
-                taken += __popc(seg);
-                thr = m;
-            }
-            if (hit) {
-#pragma unroll
-                for (int i = 0; i < 7; ++i) {
-                    h[i] = h[i + 1];
-                }
-                h[7] = FLT_MAX;
-            }
-        }
-        // survivors (distance <= thr): a row whose third smallest is above
-        // thr holds them among its stored first two; the other rows (usually
-        // the ref block's own row and a neighbour) are scanned
-        unsigned long long *sk = stage_key + wave * 64 + grp * 16;
-        uint32_t *sxy = stage_xy + wave * 64 + grp * 16;
-        uint64_t rows = 0;
-        uint32_t own = 0;  // stored survivors of this lane's rows, 2 bits per row slot
-        int cnt = 0;
-        for (int r0 = 0; r0 < rh; r0 += 16) {
-            const int r = r0 + sub;
-            bool scan = false;
-            if (act && r < rh) {
-                const float4 q = rt[r];
-                scan = q.z <= thr && q.z < FLT_MAX;
-                if (!scan) {
-                    const uint32_t s0 = (q.x <= thr && q.x < FLT_MAX) ? 1u : 0u;
-                    const uint32_t s1 = (q.y <= thr && q.y < FLT_MAX) ? 1u : 0u;
-                    own |= (s0 | (s1 << 1)) << (r0 >> 3);
-                    cnt += int(s0 + s1);
-                }
-            }
-            const uint64_t m = __ballot(scan);
-            rows |= ((m >> (grp * 16)) & 0xFFFFull) << r0;
-        }
-        const int incl = row_prefix_sum16(cnt);
-        int pos = incl - cnt;
-        const int stored = __shfl(incl, (lane & ~15) | 15);
-        for (int r0 = 0; own != 0; r0 += 16, own >>= 2) {
-            if ((own & 3u) == 0) {
-                continue;
-            }
-            const int r = r0 + sub;
-            const float4 q = rt[r];
-            const uint32_t kk = __float_as_uint(q.w);
-            if ((own & 1u) && pos < 16) {
-                const uint32_t k = kk & 0xFFFFu;
-                sk[pos] = make_key(q.x, r * rw + int(k));
-                sxy[pos] = uint32_t(left + int(k)) | (uint32_t(top + r) << 16);
-            }
-            pos += int(own & 1u);
-            if ((own & 2u) && pos < 16) {
-                const uint32_t k = kk >> 16;
-                sk[pos] = make_key(q.y, r * rw + int(k));
-                sxy[pos] = uint32_t(left + int(k)) | (uint32_t(top + r) << 16);
-            }
-            pos += int((own >> 1) & 1u);
-        }
-        int staged = stored;
-        while (rows) {
-            const int r = __ffsll((unsigned long long)rows) - 1;
-            rows &= rows - 1;
-            for (int k0 = 0; k0 < rw; k0 += 16) {
-                const int k = k0 + sub;
-                const float d = k < rw ? db[r * A.dist_cols + k] : FLT_MAX;
-                const bool surv = k < rw && d <= thr && d < FLT_MAX;
-                const uint32_t seg = uint32_t(__ballot(surv) >> (grp * 16)) & 0xFFFFu;
-                const int pos = staged + __popc(seg & ((1u << sub) - 1u));
-                if (surv && pos < 16) {
-                    sk[pos] = make_key(d, r * rw + k);
-                    sxy[pos] = uint32_t(left + k) | (uint32_t(top + r) << 16);
-                }
-                staged += __popc(seg);
-            }
-        }
-        wave_lds_sync();
-        uint64_t key = sub < staged && sub < 16 ? sk[sub] : kNoKey;
-        uint32_t kxy = sub < staged && sub < 16 ? sxy[sub] : 0u;
-        wave_lds_sync();
+        const SpatialRows geo{right - left + 1, left, top};
+        uint64_t key;
+        uint32_t kxy;
+        float thr;
+        const int staged = select16(rowtop + bs * rh, dist + bs * rh * A.dist_cols, A.dist_cols, rh, geo, act, lane,
+                                    stage_key + wave * 64 + grp * 16, stage_xy + wave * 64 + grp * 16, key, kxy, thr);
         if (act && staged <= 16) {
-            sort16_dpp(key, kxy, lane);
-            if (!(key < kPlaceholderKey)) {
-                key = kPlaceholderKey;  // fewer than 8 candidates in the window
-                kxy = 0u;
-            }
             if (A.spatial) {
                 if (sub < 8) {
                     A.spatial[(long long)(by * A.nx + bx) * 8 + sub] = make_uint2(uint32_t(key >> 32), kxy);
@@ -793,11 +907,13 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         }
         if (act && sub == 0) {
             stage_over[b] = staged > 16 ? 1 : 0;
+            stage_thr[b] = thr;
         }
     }
     __syncthreads();
 
-    // blocks with more than 16 survivors (ties): a whole wavefront each
+    // blocks with more than 16 candidates at or below the threshold (ties):
+    // a whole wavefront each
     for (int b = wave; b < nb; b += 4) {
         if (!stage_over[b]) {
             continue;
@@ -805,74 +921,10 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int bx = bx0 + b;
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
-        const int rw = right - left + 1;
-        const int n = rw * rh;
-        const float *db = dist + b * rh * A.dist_cols;
-        const float4 *rt = rowtop + b * rh;
-        // lane value: the minimum of its rows; the 8th smallest lane value
-        // bounds the 8th smallest distance from above (8 lanes each own a
-        // candidate at or below it)
-        float a = FLT_MAX;
-        for (int r = lane; r < rh; r += 64) {
-            a = fminf(a, rt[r].x);
-        }
-        const float thr = kth_smallest8(a, lane);
-        // survivors (distance <= thr) of the rows that can hold one, staged in LDS
-        unsigned long long *sk = stage_key + wave * 64;
-        uint32_t *sxy = stage_xy + wave * 64;
+        const SpatialRows geo{right - left + 1, left, top};
         TopList L;
-        list_reset(L);
-        int staged = 0;
-        bool merged = false;
-        for (int r0 = 0; r0 < rh; r0 += 64) {
-            uint64_t rows = __ballot(r0 + lane < rh && rt[r0 + lane].x <= thr);
-            while (rows) {
-                const int r = r0 + __ffsll((unsigned long long)rows) - 1;
-                rows &= rows - 1;
-                for (int k0 = 0; k0 < rw; k0 += 64) {
-                    const int k = k0 + lane;
-                    const float d = k < rw ? db[r * A.dist_cols + k] : FLT_MAX;
-                    const bool surv = k < rw && d <= thr && d < FLT_MAX;
-                    const uint64_t m = __ballot(surv);
-                    const int cnt = __popcll(m);
-                    if (staged + cnt > 64) {
-                        // rare (many ties): fold the stage into the list first
-                        wave_lds_sync();
-                        const uint64_t key = lane < staged ? sk[lane] : kNoKey;
-                        const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
-                        wave_lds_sync();
-                        list_merge_batch(L, key, kxy, lane);
-                        staged = 0;
-                        merged = true;
-                    }
-                    if (surv) {
-                        const int pos = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
-                                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
-                        sk[pos] = make_key(d, r * rw + k);
-                        sxy[pos] = uint32_t(left + k) | (uint32_t(top + r) << 16);
-                    }
-                    staged += cnt;
-                }
-            }
-        }
-        wave_lds_sync();
-        {
-            const uint64_t key = lane < staged ? sk[lane] : kNoKey;
-            const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
-            if (!merged && staged <= 16) {
-                // the usual case: sort the <= 16 survivors inside one DPP row
-                uint64_t k16 = key;
-                uint32_t x16 = kxy;
-                sort16_dpp(k16, x16, lane);
-                if (lane < 8) {
-                    L.key = k16 < kPlaceholderKey ? k16 : kPlaceholderKey;
-                    L.xy = k16 < kPlaceholderKey ? x16 : 0u;
-                }
-            } else {
-                list_merge_batch(L, key, kxy, lane);
-            }
-        }
-        wave_lds_sync();
+        select_wave(rowtop + b * rh, dist + b * rh * A.dist_cols, A.dist_cols, rh, geo, stage_thr[b], lane,
+                    stage_key + wave * 64, stage_xy + wave * 64, L);
         if (A.spatial) {
             if (lane < 8) {
                 A.spatial[(long long)(by * A.nx + bx) * 8 + lane] = make_uint2(uint32_t(L.key >> 32), L.xy);
@@ -883,6 +935,238 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
             G.xy = L.xy;
             G.z = A.radius;
             finish_group(A, G, by * A.nx + bx, xb, y, lane);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Temporal predictive search of the sliding metric (radius 1 or 2), after
+// bm3d_match_rows has stored each ref block's spatial top-8.  A workgroup
+// takes t_blocks consecutive ref blocks.  The two directions are independent
+// chains (frames r-1, r-2, .. and r+1, r+2, ..), so step t searches frames
+// r-t and r+t together: the ps_num predictor windows of every (block,
+// direction) are staged in LDS, every (block, direction, window, row) is one
+// thread of the row scan, and every (block, direction) one 16-lane
+// selection of the frame's top-8, whose positions are the next step's
+// predictors (src/bm3d.comp:540-637).  The group list then takes the first
+// ps_num entries of each frame's top-8 in the shader's order (direction -1
+// for t = 1..r, then +1) and the self-block rule (src/bm3d.comp:626-663).
+// ---------------------------------------------------------------------------
+template <int CM>
+__global__ __launch_bounds__(256) void bm3d_match_temporal(MatchArgs A)
+{
+    extern __shared__ __align__(16) unsigned long long lds64[];
+    const int NBT = A.t_blocks, PS = A.ps_num, PR = A.ps_range, r = A.radius;
+    const int WH = 2 * PR + 1;        // rows (and columns) of a full window
+    const int CSW = WH + 7;           // column stride of a staged window
+    const int WCAP = CSW * (WH + 7);  // floats of a staged window
+    const int NW = 2 * PS;            // windows of one block per step
+    const int RPC = PS * WH;          // row slots of one direction
+    const int RPB = 2 * RPC;          // row slots of one block
+    const int win = WH * WH;
+
+    unsigned long long *stage_key = lds64;                                      // [4][64]
+    uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);             // [4][64]
+    unsigned long long *fkey = reinterpret_cast<unsigned long long *>(stage_xy + 256);  // [NBT][2][8]
+    float4 *rowtop = reinterpret_cast<float4 *>(fkey + NBT * 16);               // [NBT][RPB]
+    int4 *geo = reinterpret_cast<int4 *>(rowtop + NBT * RPB);                   // [NBT][2][PS]
+    float *region = reinterpret_cast<float *>(geo + NBT * NW);                  // [NBT][2][PS][WCAP]
+    float *cur = region + NBT * NW * WCAP;                                      // [NBT][64]
+    float *dist = cur + NBT * 64;                                               // [NBT][RPB][WH]
+    uint32_t *pred = reinterpret_cast<uint32_t *>(dist + NBT * RPB * WH);      // [NBT][2][8]
+    uint32_t *fxy = pred + NBT * 16;                                            // [NBT][2][8]
+    uint32_t *fsave = fxy + NBT * 16;                                           // [NBT][2][2][PS][2]
+    int *over = reinterpret_cast<int *>(fsave + NBT * 2 * 2 * PS * 2);         // [NBT][2]
+    float *othr = reinterpret_cast<float *>(over + NBT * 2);                    // [NBT][2]
+
+    const int W = A.width, H = A.height, pitch = A.pitch;
+    const int nref = A.nx * A.ny;
+    const int blk0 = xcd_block_id() * NBT;
+    const int nbt = min(NBT, nref - blk0);
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+
+    // the ref blocks (cur[c*8+i] = pixel (x+c, y+i)) and, for both
+    // directions, the spatial top-8 positions as the first predictors
+    for (int i = tid; i < nbt * 64; i += 256) {
+        const int b = i >> 6, e = i & 63;
+        const int blk = blk0 + b;
+        const int by = blk / A.nx, bx = blk - by * A.nx;
+        const int x = min(bx * A.step, W - 8), y = min(by * A.step, H - 8);
+        cur[i] = A.plane[r][(long long)(y + (e & 7)) * pitch + x + (e >> 3)];
+    }
+    for (int i = tid; i < nbt * 16; i += 256) {
+        const int b = i >> 4, j = i & 7;
+        pred[i] = A.spatial[(long long)(blk0 + b) * 8 + j].y;
+    }
+    __syncthreads();
+
+    for (int t = 1; t <= r; ++t) {
+        // window geometry around each predictor
+        if (tid < nbt * NW) {
+            const int b = tid / NW, cw = tid - b * NW, c = cw / PS, w = cw - c * PS;
+            const uint32_t p = pred[(b * 2 + c) * 8 + w];
+            const int cx = int(p & 0xFFFFu), cy = int(p >> 16);
+            const int left = max(cx - PR, 0), right = min(cx + PR, W - 8);
+            const int top = max(cy - PR, 0), bottom = min(cy + PR, H - 8);
+            geo[tid] = make_int4(left, top, right - left + 1, bottom - top + 1);
+        }
+        __syncthreads();
+        // stage the windows (rows top .. bottom+7, columns left .. right+7)
+        // of frame r-t (direction -1) and r+t (+1), column-major
+        for (int wi = wave; wi < nbt * NW; wi += 4) {
+            const int4 g = geo[wi];
+            const int c = (wi % NW) / PS;
+            const float *plane = A.plane[c == 0 ? r - t : r + t];
+            const int ncols = g.z + 7, nrows = g.w + 7;
+            const int per = 64 / ncols;  // ncols <= 2*ps_range + 8 <= 64
+            const int rs = lane / ncols, col = lane - rs * ncols;
+            float *reg = region + wi * WCAP;
+            if (rs < per) {
+                for (int row = rs; row < nrows; row += per) {
+                    reg[col * CSW + row] = plane[(long long)(g.y + row) * pitch + g.x + col];
+                }
+            }
+        }
+        __syncthreads();
+        // row scans: one (block, direction, window, row) per thread
+        if (tid < nbt * RPB) {
+            const int b = tid / RPB, rho = tid - b * RPB;
+            const int c = rho / RPC, w = (rho - c * RPC) / WH, rr = rho - c * RPC - w * WH;
+            const int wi = b * NW + c * PS + w;
+            const int4 g = geo[wi];
+            float4 q = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.0f);  // empty row slot
+            if (rr < g.w) {
+                q = scan_row_sliding<CM>(cur + b * 64, region + wi * WCAP + rr, CSW, g.z,
+                                         dist + (b * RPB + rho) * WH);
+            }
+            rowtop[b * RPB + rho] = q;
+        }
+        __syncthreads();
+        // the frame top-8 of each (block, direction) by a 16-lane row
+        for (int u0 = 0; u0 < nbt * 2; u0 += 16) {
+            const int grp = lane >> 4, sub = lane & 15;
+            const int u = u0 + wave + 4 * grp;
+            const bool act = u < nbt * 2;
+            const int us = act ? u : 0;
+            const int b = us >> 1, c = us & 1;
+            const PredictiveRows pg{geo + b * NW + c * PS, WH, win};
+            uint64_t key;
+            uint32_t kxy;
+            float thr;
+            const int staged = select16(rowtop + b * RPB + c * RPC, dist + (b * RPB + c * RPC) * WH, WH, RPC, pg,
+                                        act, lane, stage_key + wave * 64 + grp * 16,
+                                        stage_xy + wave * 64 + grp * 16, key, kxy, thr);
+            if (act && staged <= 16 && sub < 8) {
+                fkey[u * 8 + sub] = key;
+                fxy[u * 8 + sub] = kxy;
+            }
+            if (act && sub == 0) {
+                over[u] = staged > 16 ? 1 : 0;
+                othr[u] = thr;
+            }
+        }
+        __syncthreads();
+        for (int u = wave; u < nbt * 2; u += 4) {
+            if (!over[u]) {
+                continue;
+            }
+            const int b = u >> 1, c = u & 1;
+            const PredictiveRows pg{geo + b * NW + c * PS, WH, win};
+            TopList L;
+            select_wave(rowtop + b * RPB + c * RPC, dist + (b * RPB + c * RPC) * WH, WH, RPC, pg, othr[u], lane,
+                        stage_key + wave * 64, stage_xy + wave * 64, L);
+            if (lane < 8) {
+                fkey[u * 8 + lane] = L.key;
+                fxy[u * 8 + lane] = L.xy;
+            }
+        }
+        __syncthreads();
+        // next predictors; the first ps_num entries kept for the group list
+        for (int i = tid; i < nbt * 16; i += 256) {
+            const int u = i >> 3, j = i & 7;
+            pred[i] = fxy[i];
+            if (j < PS) {
+                uint32_t *sv = fsave + ((u * 2 + (t - 1)) * PS + j) * 2;
+                sv[0] = uint32_t(fkey[i] >> 32);
+                sv[1] = fxy[i];
+            }
+        }
+        __syncthreads();
+    }
+
+    // group list: spatial top-8, then the temporal inserts in the shader's
+    // order, then the self-block rule; one thread per block
+    if (tid < nbt) {
+        const int blk = blk0 + tid;
+        float e[8];
+        uint32_t xy[8];
+        int z[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint2 sp = A.spatial[(long long)blk * 8 + j];
+            e[j] = __uint_as_float(sp.x);
+            xy[j] = sp.y;
+            z[j] = r;
+        }
+        for (int c = 0; c < 2; ++c) {
+            for (int t = 1; t <= r; ++t) {
+                const int zz = c == 0 ? r - t : r + t;
+                for (int i = 0; i < PS; ++i) {
+                    const uint32_t *sv = fsave + (((tid * 2 + c) * 2 + (t - 1)) * PS + i) * 2;
+                    const float fe = __uint_as_float(sv[0]);
+                    const uint32_t fp = sv[1];
+                    if (!(fe < e[7])) {
+                        continue;  // ginsert8: strict '<', after equal ones
+                    }
+                    int pos = 0;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        pos += e[j] <= fe ? 1 : 0;
+                    }
+#pragma unroll
+                    for (int j = 7; j > 0; --j) {
+                        if (j > pos) {
+                            e[j] = e[j - 1];
+                            xy[j] = xy[j - 1];
+                            z[j] = z[j - 1];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        if (j == pos) {
+                            e[j] = fe;
+                            xy[j] = fp;
+                            z[j] = zz;
+                        }
+                    }
+                }
+            }
+        }
+        const int by = blk / A.nx, bx = blk - by * A.nx;
+        const int x = min(bx * A.step, W - 8), y = min(by * A.step, H - 8);
+        const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
+        bool present = false;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            present = present || (xy[j] == self_xy && z[j] == r);
+        }
+        if (!present) {
+#pragma unroll
+            for (int j = 7; j > 0; --j) {
+                e[j] = e[j - 1];
+                xy[j] = xy[j - 1];
+                z[j] = z[j - 1];
+            }
+            e[0] = -1.0f;
+            xy[0] = self_xy;
+            z[0] = r;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            A.groups[(long long)blk * 8 + j] = pack_entry(xy[j] & 0xFFFFu, xy[j] >> 16, uint32_t(z[j]));
+            if (A.dists) {
+                A.dists[(long long)blk * 8 + j] = e[j];
+            }
         }
     }
 }
@@ -1515,6 +1799,16 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
             hipLaunchKernelGGL((bm3d_match_rows<kFpExplicit>), rgrid, block, lds, s, rows);
         }
         if (a.radius == 0) {
+            return hipGetLastError();
+        }
+        if (a.t_blocks > 0) {
+            const size_t tlds = size_t(temporal_lds_bytes(a.t_blocks, a.ps_num, a.ps_range));
+            const dim3 tgrid((nref + a.t_blocks - 1) / a.t_blocks);
+            if (fused) {
+                hipLaunchKernelGGL((bm3d_match_temporal<kFpAll>), tgrid, block, tlds, s, a);
+            } else {
+                hipLaunchKernelGGL((bm3d_match_temporal<kFpExplicit>), tgrid, block, tlds, s, a);
+            }
             return hipGetLastError();
         }
         if (fused) {

@@ -158,6 +158,10 @@ class Instance:
         """Record HIP events around every kernel launch (resets the totals)."""
         _lib.load().vsfeel_bm3d_set_timing(self.handle, int(bool(enable)))
 
+    def set_graphs(self, enable: bool) -> None:
+        """Replay process() from a captured HIP graph while its arguments repeat (default on)."""
+        _lib.load().vsfeel_bm3d_set_graphs(self.handle, int(bool(enable)))
+
     def kernel_times(self) -> dict:
         """{kernel: (total ms, launches)} of the launches recorded so far."""
         n = len(_lib.KERNEL_NAMES)
