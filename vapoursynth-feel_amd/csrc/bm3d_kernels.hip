@@ -518,78 +518,64 @@ __device__ __forceinline__ float4 scan_row_sliding(const float *cb, const float 
     }
     float m1 = FLT_MAX, m2 = FLT_MAX, m3 = FLT_MAX;
     uint32_t k1 = 0, k2 = 0;
-    float v[15];
-    for (int k0 = 0; k0 < rw; k0 += 8) {
-        if (k0 == 0) {
+    // Column by column with the partial sums of the tree kept rolling:
+    // p1_i = v_i + v_i+1, p2_i = p1_i + p1_i+2, D_k = p2_k + p2_k+4.  Column j
+    // completes p1_j-1, p2_j-3 and, from j = 7 on, D_j-7.
+    float v_prev = 0.0f;
+    float p1_2 = 0.0f, p1_3 = 0.0f;                          // p1_j-2, p1_j-3
+    float p2_4 = 0.0f, p2_5 = 0.0f, p2_6 = 0.0f, p2_7 = 0.0f;  // p2_j-4 .. p2_j-7
 #pragma unroll
-            for (int j = 0; j < 15; ++j) {
-                if (j <= rw + 6) {
-                    const float *sp = col0 + j * CS;
-                    float sv[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        sv[i] = sp[i];
-                    }
-                    if (j < 7) {
-                        float cc[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            cc[i] = cb[j * 8 + i];
-                        }
-                        v[j] = col_ssd<CM>(cc, sv);
-                    } else {
-                        v[j] = col_ssd<CM>(c7, sv);
-                    }
-                } else {
-                    v[j] = 0.0f;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 7; ++j) {
-                v[j] = v[j + 8];
-            }
-#pragma unroll
-            for (int j = 7; j < 15; ++j) {
-                if (k0 + j <= rw + 6) {
-                    const float *sp = col0 + (k0 + j) * CS;
-                    float sv[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        sv[i] = sp[i];
-                    }
-                    v[j] = col_ssd<CM>(c7, sv);
-                } else {
-                    v[j] = 0.0f;
-                }
-            }
-        }
-        float p1[14], p2[12];
-#pragma unroll
-        for (int j = 0; j < 14; ++j) {
-            p1[j] = v[j] + v[j + 1];
-        }
-#pragma unroll
-        for (int j = 0; j < 12; ++j) {
-            p2[j] = p1[j] + p1[j + 2];
-        }
+    for (int j = 0; j < 7; ++j) {
+        // the first 7 columns of the row against reference columns 0..6
+        const float *sp = col0 + j * CS;
+        float sv[8], cc[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            if (k0 + i < rw) {
-                const float d = p2[i] + p2[i + 4];
-                drow[k0 + i] = d;
-                // FLT_MAX and NaN never count (the shader's strict '<')
-                const float dv = (d < FLT_MAX) ? d : FLT_MAX;
-                const bool c1 = dv < m1, c2 = dv < m2;
-                k2 = c1 ? k1 : (c2 ? uint32_t(k0 + i) : k2);
-                k1 = c1 ? uint32_t(k0 + i) : k1;
-                const float t1 = fmaxf(m1, dv);
-                m1 = fminf(m1, dv);
-                const float t2 = fmaxf(m2, t1);
-                m2 = fminf(m2, t1);
-                m3 = fminf(m3, t2);
-            }
+            sv[i] = sp[i];
+            cc[i] = cb[j * 8 + i];
         }
+        const float v = col_ssd<CM>(cc, sv);
+        const float p1 = v_prev + v;    // p1_j-1 (j >= 1)
+        const float p2 = p1_3 + p1;     // p2_j-3 (j >= 3)
+        v_prev = v;
+        p1_3 = p1_2;
+        p1_2 = p1;
+        p2_7 = p2_6;
+        p2_6 = p2_5;
+        p2_5 = p2_4;
+        p2_4 = p2;
+    }
+#pragma unroll 4
+    for (int j = 7; j <= rw + 6; ++j) {
+        const float *sp = col0 + j * CS;
+        float sv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            sv[i] = sp[i];
+        }
+        const float v = col_ssd<CM>(c7, sv);
+        const float p1 = v_prev + v;
+        const float p2 = p1_3 + p1;
+        const float d = p2_7 + p2;      // D_j-7
+        v_prev = v;
+        p1_3 = p1_2;
+        p1_2 = p1;
+        p2_7 = p2_6;
+        p2_6 = p2_5;
+        p2_5 = p2_4;
+        p2_4 = p2;
+        const uint32_t k = uint32_t(j - 7);
+        drow[k] = d;
+        // FLT_MAX and NaN never count (the shader's strict '<')
+        const float dv = (d < FLT_MAX) ? d : FLT_MAX;
+        const bool c1 = dv < m1, c2 = dv < m2;
+        k2 = c1 ? k1 : (c2 ? k : k2);
+        k1 = c1 ? k : k1;
+        const float t1 = fmaxf(m1, dv);
+        m1 = fminf(m1, dv);
+        const float t2 = fmaxf(m2, t1);
+        m2 = fminf(m2, t1);
+        m3 = fminf(m3, t2);
     }
     return make_float4(m1, m2, m3, __uint_as_float(k1 | (k2 << 16)));
 }
@@ -603,21 +589,30 @@ struct SpatialRows {
     __device__ uint32_t xy(int row, int k) const { return uint32_t(left + k) | (uint32_t(top + row) << 16); }
 };
 
+constexpr int kStageBatch = 8;  // loads a thread keeps in flight while staging
+
+// floor(n / d) for 0 <= n < 2^16 and d >= 1, given inv = 1.0f / d: (n + 1/2)
+// / d lies at least 1/(2d) from an integer, while the float product is off
+// by at most 2^-23 (n + 1/2) / d < 2^-7 / d, so truncation gives the exact
+// quotient
+__device__ __forceinline__ int div_small(int n, float inv) { return int((float(n) + 0.5f) * inv); }
+
 // The ps_num predictor windows of one temporal step: row slot rho = w*wh + r
 // (rows r >= the window's height are empty), seq = w*win + r*rw_w + k
 // (src/bm3d.comp:569-590).
 struct PredictiveRows {
     const int4 *geo;  // per window: left, top, rw, rh
     int wh, win;
-    __device__ int len(int rho) const { return geo[rho / wh].z; }
+    float inv_wh;
+    __device__ int len(int rho) const { return geo[div_small(rho, inv_wh)].z; }
     __device__ int seq(int rho, int k) const
     {
-        const int w = rho / wh;
+        const int w = div_small(rho, inv_wh);
         return w * win + (rho - w * wh) * geo[w].z + k;
     }
     __device__ uint32_t xy(int rho, int k) const
     {
-        const int w = rho / wh;
+        const int w = div_small(rho, inv_wh);
         const int4 g = geo[w];
         return uint32_t(g.x + k) | (uint32_t(g.y + rho - w * wh) << 16);
     }
@@ -856,9 +851,31 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 
     // stage the source region (rows top .. bottom+7, columns rx0 .. rx1)
     const int ncols = rx1 - rx0 + 1, nrows = rh + 7;
-    for (int i = threadIdx.x; i < ncols * nrows; i += 256) {
-        const int row = i / ncols, col = i - row * ncols;
-        region[col * CS + row] = plane[(long long)(top + row) * pitch + rx0 + col];
+    {
+        // every thread issues its batch of loads before storing any of them
+        const int n = ncols * nrows;
+        const float inv = 1.0f / float(ncols);
+        for (int i0 = threadIdx.x; i0 < n; i0 += 256 * kStageBatch) {
+            float v[kStageBatch];
+            int at[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = i0 + u * 256;
+                at[u] = -1;
+                v[u] = 0.0f;
+                if (i < n) {
+                    const int row = div_small(i, inv), col = i - row * ncols;
+                    v[u] = plane[(long long)(top + row) * pitch + rx0 + col];
+                    at[u] = col * CS + row;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                if (at[u] >= 0) {
+                    region[at[u]] = v[u];
+                }
+            }
+        }
     }
     for (int i = threadIdx.x; i < nb * 64; i += 256) {
         const int b = i >> 6, e = i & 63;
@@ -964,6 +981,8 @@ __global__ __launch_bounds__(256) void bm3d_match_temporal(MatchArgs A)
     const int RPC = PS * WH;          // row slots of one direction
     const int RPB = 2 * RPC;          // row slots of one block
     const int win = WH * WH;
+    const float inv_wh = 1.0f / float(WH), inv_csw = 1.0f / float(CSW);
+    const float inv_wcap = 1.0f / float(WCAP), inv_nw = 1.0f / float(NW);
 
     unsigned long long *stage_key = lds64;                                      // [4][64]
     uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);             // [4][64]
@@ -1013,17 +1032,38 @@ __global__ __launch_bounds__(256) void bm3d_match_temporal(MatchArgs A)
         __syncthreads();
         // stage the windows (rows top .. bottom+7, columns left .. right+7)
         // of frame r-t (direction -1) and r+t (+1), column-major
-        for (int wi = wave; wi < nbt * NW; wi += 4) {
-            const int4 g = geo[wi];
-            const int c = (wi % NW) / PS;
-            const float *plane = A.plane[c == 0 ? r - t : r + t];
-            const int ncols = g.z + 7, nrows = g.w + 7;
-            const int per = 64 / ncols;  // ncols <= 2*ps_range + 8 <= 64
-            const int rs = lane / ncols, col = lane - rs * ncols;
-            float *reg = region + wi * WCAP;
-            if (rs < per) {
-                for (int row = rs; row < nrows; row += per) {
-                    reg[col * CSW + row] = plane[(long long)(g.y + row) * pitch + g.x + col];
+        {
+            // element e of the staged windows: window e / WCAP, then row-major
+            // in a (wh+7) x (wh+7) frame; every thread issues its batch of
+            // loads before storing any of them
+            const float *pm = A.plane[r - t], *pp = A.plane[r + t];
+            const int n = nbt * NW * WCAP;
+            for (int e0 = tid; e0 < n; e0 += 256 * kStageBatch) {
+                float v[kStageBatch];
+                int at[kStageBatch];
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int e = e0 + u * 256;
+                    at[u] = -1;
+                    v[u] = 0.0f;
+                    if (e < n) {
+                        const int wi = div_small(e, inv_wcap);
+                        const int rem = e - wi * WCAP;
+                        const int row = div_small(rem, inv_csw), col = rem - row * CSW;
+                        const int4 g = geo[wi];
+                        if (row < g.w + 7 && col < g.z + 7) {
+                            const int cw = wi - div_small(wi, inv_nw) * NW;  // c * PS + w
+                            const float *plane = cw < PS ? pm : pp;
+                            v[u] = plane[(long long)(g.y + row) * pitch + g.x + col];
+                            at[u] = wi * WCAP + col * CSW + row;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kStageBatch; ++u) {
+                    if (at[u] >= 0) {
+                        region[at[u]] = v[u];
+                    }
                 }
             }
         }
@@ -1049,7 +1089,7 @@ __global__ __launch_bounds__(256) void bm3d_match_temporal(MatchArgs A)
             const bool act = u < nbt * 2;
             const int us = act ? u : 0;
             const int b = us >> 1, c = us & 1;
-            const PredictiveRows pg{geo + b * NW + c * PS, WH, win};
+            const PredictiveRows pg{geo + b * NW + c * PS, WH, win, inv_wh};
             uint64_t key;
             uint32_t kxy;
             float thr;
@@ -1071,7 +1111,7 @@ __global__ __launch_bounds__(256) void bm3d_match_temporal(MatchArgs A)
                 continue;
             }
             const int b = u >> 1, c = u & 1;
-            const PredictiveRows pg{geo + b * NW + c * PS, WH, win};
+            const PredictiveRows pg{geo + b * NW + c * PS, WH, win, inv_wh};
             TopList L;
             select_wave(rowtop + b * RPB + c * RPC, dist + (b * RPB + c * RPC) * WH, WH, RPC, pg, othr[u], lane,
                         stage_key + wave * 64, stage_xy + wave * 64, L);
