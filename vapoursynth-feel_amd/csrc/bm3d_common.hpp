@@ -35,8 +35,6 @@ struct MatchArgs {
     // layout (column-major source region, per-block distance rows)
     int run_blocks, runs_x;
     int region_cols, region_rows, dist_cols;
-    // bm3d_match_temporal: ref blocks per workgroup (0: bm3d_match searches)
-    int t_blocks;
 };
 
 // LDS bytes of bm3d_match_rows: survivor stages of its 4 wavefronts (64 keys
@@ -50,22 +48,6 @@ __host__ __device__ inline long long rows_lds_bytes(int region_cols, int region_
     return 256LL * 8 + 256LL * 4 + 16LL * run_blocks * rh +
            4LL * ((long long)region_cols * region_rows + run_blocks * 64LL + run_blocks * rh * dist_cols +
                   2LL * run_blocks);
-}
-
-// LDS bytes of bm3d_match_temporal for t_blocks ref blocks: survivor stages,
-// per (block, direction) its frame top-8 (keys, positions) and the next
-// predictors, per window row slot its 3 smallest distances, per window its
-// geometry and staged source region ((2 ps_range + 8)^2 floats), the blocks,
-// the distance rows, per (block, direction, step) the ps_num entries kept for
-// the group list (radius <= 2), and per (block, direction) an overflow flag
-// and its threshold.
-__host__ __device__ inline long long temporal_lds_bytes(int t_blocks, int ps_num, int ps_range)
-{
-    const long long nbt = t_blocks, wh = 2LL * ps_range + 1, nw = 2LL * ps_num, rpb = 2LL * ps_num * wh;
-    const long long wcap = (wh + 7) * (wh + 7);
-    return 256LL * 8 + 256LL * 4 + nbt * 16 * 8 + nbt * rpb * 16 + nbt * nw * 16 +
-           4LL * (nbt * nw * wcap + nbt * 64 + nbt * rpb * wh + nbt * 16 + nbt * 16 + nbt * 2 * 2 * ps_num * 2 +
-                  nbt * 2 * 2);
 }
 
 // Collaborative filtering + weighted scatter of every group.

@@ -128,9 +128,6 @@ struct Instance {
     // workgroup run and the LDS layout; run_blocks == 0 selects the
     // per-wavefront search instead
     int run_blocks = 0, runs_x = 0, region_cols = 0, region_rows = 0, dist_cols = 0;
-    // row-parallel temporal search (bm3d_match_temporal): ref blocks per
-    // workgroup; 0 selects the per-wavefront search
-    int t_blocks = 0;
     float *dst_dev = nullptr;    // one plane
 
     // tiled accumulation of the frame's own slice (see bm3d_estimate_tiled)
@@ -310,7 +307,6 @@ struct Instance {
         m.region_cols = region_cols;
         m.region_rows = region_rows;
         m.dist_cols = dist_cols;
-        m.t_blocks = t_blocks;
         return m;
     }
 
@@ -615,19 +611,6 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
                 d.region_cols = cols;
                 d.region_rows = rh + 7;
                 d.dist_cols = dcols;
-            }
-        }
-        if (d.run_blocks > 0 && d.radius > 0) {
-            // temporal search: one thread per (block, direction, predictor
-            // window, window row); a staged window row of at most 64 columns
-            const int wh = 2 * d.ps_range + 1;
-            const int rpb = 2 * d.ps_num * wh;
-            if (wh + 7 <= 64 && rpb <= 256) {
-                int nbt = 256 / rpb;
-                while (nbt > 0 && temporal_lds_bytes(nbt, d.ps_num, d.ps_range) > 64 * 1024) {
-                    --nbt;
-                }
-                d.t_blocks = nbt;
             }
         }
     }

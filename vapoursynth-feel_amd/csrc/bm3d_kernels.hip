@@ -597,27 +597,6 @@ constexpr int kStageBatch = 8;  // loads a thread keeps in flight while staging
 // quotient
 __device__ __forceinline__ int div_small(int n, float inv) { return int((float(n) + 0.5f) * inv); }
 
-// The ps_num predictor windows of one temporal step: row slot rho = w*wh + r
-// (rows r >= the window's height are empty), seq = w*win + r*rw_w + k
-// (src/bm3d.comp:569-590).
-struct PredictiveRows {
-    const int4 *geo;  // per window: left, top, rw, rh
-    int wh, win;
-    float inv_wh;
-    __device__ int len(int rho) const { return geo[div_small(rho, inv_wh)].z; }
-    __device__ int seq(int rho, int k) const
-    {
-        const int w = div_small(rho, inv_wh);
-        return w * win + (rho - w * wh) * geo[w].z + k;
-    }
-    __device__ uint32_t xy(int rho, int k) const
-    {
-        const int w = div_small(rho, inv_wh);
-        const int4 g = geo[w];
-        return uint32_t(g.x + k) | (uint32_t(g.y + rho - w * wh) << 16);
-    }
-};
-
 // ---------------------------------------------------------------------------
 // Top-8 (distance, seq) keys of one search by a 16-lane row of the wavefront
 // (sub = lane & 15), from the 3 smallest distances of each of its nrows rows
@@ -952,261 +931,6 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
             G.xy = L.xy;
             G.z = A.radius;
             finish_group(A, G, by * A.nx + bx, xb, y, lane);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Temporal predictive search of the sliding metric (radius 1 or 2), after
-// bm3d_match_rows has stored each ref block's spatial top-8.  A workgroup
-// takes t_blocks consecutive ref blocks.  The two directions are independent
-// chains (frames r-1, r-2, .. and r+1, r+2, ..), so step t searches frames
-// r-t and r+t together: the ps_num predictor windows of every (block,
-// direction) are staged in LDS, every (block, direction, window, row) is one
-// thread of the row scan, and every (block, direction) one 16-lane
-// selection of the frame's top-8, whose positions are the next step's
-// predictors (src/bm3d.comp:540-637).  The group list then takes the first
-// ps_num entries of each frame's top-8 in the shader's order (direction -1
-// for t = 1..r, then +1) and the self-block rule (src/bm3d.comp:626-663).
-// ---------------------------------------------------------------------------
-template <int CM>
-__global__ __launch_bounds__(256) void bm3d_match_temporal(MatchArgs A)
-{
-    extern __shared__ __align__(16) unsigned long long lds64[];
-    const int NBT = A.t_blocks, PS = A.ps_num, PR = A.ps_range, r = A.radius;
-    const int WH = 2 * PR + 1;        // rows (and columns) of a full window
-    const int CSW = WH + 7;           // column stride of a staged window
-    const int WCAP = CSW * (WH + 7);  // floats of a staged window
-    const int NW = 2 * PS;            // windows of one block per step
-    const int RPC = PS * WH;          // row slots of one direction
-    const int RPB = 2 * RPC;          // row slots of one block
-    const int win = WH * WH;
-    const float inv_wh = 1.0f / float(WH), inv_csw = 1.0f / float(CSW);
-    const float inv_wcap = 1.0f / float(WCAP), inv_nw = 1.0f / float(NW);
-
-    unsigned long long *stage_key = lds64;                                      // [4][64]
-    uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);             // [4][64]
-    unsigned long long *fkey = reinterpret_cast<unsigned long long *>(stage_xy + 256);  // [NBT][2][8]
-    float4 *rowtop = reinterpret_cast<float4 *>(fkey + NBT * 16);               // [NBT][RPB]
-    int4 *geo = reinterpret_cast<int4 *>(rowtop + NBT * RPB);                   // [NBT][2][PS]
-    float *region = reinterpret_cast<float *>(geo + NBT * NW);                  // [NBT][2][PS][WCAP]
-    float *cur = region + NBT * NW * WCAP;                                      // [NBT][64]
-    float *dist = cur + NBT * 64;                                               // [NBT][RPB][WH]
-    uint32_t *pred = reinterpret_cast<uint32_t *>(dist + NBT * RPB * WH);      // [NBT][2][8]
-    uint32_t *fxy = pred + NBT * 16;                                            // [NBT][2][8]
-    uint32_t *fsave = fxy + NBT * 16;                                           // [NBT][2][2][PS][2]
-    int *over = reinterpret_cast<int *>(fsave + NBT * 2 * 2 * PS * 2);         // [NBT][2]
-    float *othr = reinterpret_cast<float *>(over + NBT * 2);                    // [NBT][2]
-
-    const int W = A.width, H = A.height, pitch = A.pitch;
-    const int nref = A.nx * A.ny;
-    const int blk0 = xcd_block_id() * NBT;
-    const int nbt = min(NBT, nref - blk0);
-    const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-
-    // the ref blocks (cur[c*8+i] = pixel (x+c, y+i)) and, for both
-    // directions, the spatial top-8 positions as the first predictors
-    for (int i = tid; i < nbt * 64; i += 256) {
-        const int b = i >> 6, e = i & 63;
-        const int blk = blk0 + b;
-        const int by = blk / A.nx, bx = blk - by * A.nx;
-        const int x = min(bx * A.step, W - 8), y = min(by * A.step, H - 8);
-        cur[i] = A.plane[r][(long long)(y + (e & 7)) * pitch + x + (e >> 3)];
-    }
-    for (int i = tid; i < nbt * 16; i += 256) {
-        const int b = i >> 4, j = i & 7;
-        pred[i] = A.spatial[(long long)(blk0 + b) * 8 + j].y;
-    }
-    __syncthreads();
-
-    for (int t = 1; t <= r; ++t) {
-        // window geometry around each predictor
-        if (tid < nbt * NW) {
-            const int b = tid / NW, cw = tid - b * NW, c = cw / PS, w = cw - c * PS;
-            const uint32_t p = pred[(b * 2 + c) * 8 + w];
-            const int cx = int(p & 0xFFFFu), cy = int(p >> 16);
-            const int left = max(cx - PR, 0), right = min(cx + PR, W - 8);
-            const int top = max(cy - PR, 0), bottom = min(cy + PR, H - 8);
-            geo[tid] = make_int4(left, top, right - left + 1, bottom - top + 1);
-        }
-        __syncthreads();
-        // stage the windows (rows top .. bottom+7, columns left .. right+7)
-        // of frame r-t (direction -1) and r+t (+1), column-major
-        {
-            // element e of the staged windows: window e / WCAP, then row-major
-            // in a (wh+7) x (wh+7) frame; every thread issues its batch of
-            // loads before storing any of them
-            const float *pm = A.plane[r - t], *pp = A.plane[r + t];
-            const int n = nbt * NW * WCAP;
-            for (int e0 = tid; e0 < n; e0 += 256 * kStageBatch) {
-                float v[kStageBatch];
-                int at[kStageBatch];
-#pragma unroll
-                for (int u = 0; u < kStageBatch; ++u) {
-                    const int e = e0 + u * 256;
-                    at[u] = -1;
-                    v[u] = 0.0f;
-                    if (e < n) {
-                        const int wi = div_small(e, inv_wcap);
-                        const int rem = e - wi * WCAP;
-                        const int row = div_small(rem, inv_csw), col = rem - row * CSW;
-                        const int4 g = geo[wi];
-                        if (row < g.w + 7 && col < g.z + 7) {
-                            const int cw = wi - div_small(wi, inv_nw) * NW;  // c * PS + w
-                            const float *plane = cw < PS ? pm : pp;
-                            v[u] = plane[(long long)(g.y + row) * pitch + g.x + col];
-                            at[u] = wi * WCAP + col * CSW + row;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < kStageBatch; ++u) {
-                    if (at[u] >= 0) {
-                        region[at[u]] = v[u];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // row scans: one (block, direction, window, row) per thread
-        if (tid < nbt * RPB) {
-            const int b = tid / RPB, rho = tid - b * RPB;
-            const int c = rho / RPC, w = (rho - c * RPC) / WH, rr = rho - c * RPC - w * WH;
-            const int wi = b * NW + c * PS + w;
-            const int4 g = geo[wi];
-            float4 q = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.0f);  // empty row slot
-            if (rr < g.w) {
-                q = scan_row_sliding<CM>(cur + b * 64, region + wi * WCAP + rr, CSW, g.z,
-                                         dist + (b * RPB + rho) * WH);
-            }
-            rowtop[b * RPB + rho] = q;
-        }
-        __syncthreads();
-        // the frame top-8 of each (block, direction) by a 16-lane row
-        for (int u0 = 0; u0 < nbt * 2; u0 += 16) {
-            const int grp = lane >> 4, sub = lane & 15;
-            const int u = u0 + wave + 4 * grp;
-            const bool act = u < nbt * 2;
-            const int us = act ? u : 0;
-            const int b = us >> 1, c = us & 1;
-            const PredictiveRows pg{geo + b * NW + c * PS, WH, win, inv_wh};
-            uint64_t key;
-            uint32_t kxy;
-            float thr;
-            const int staged = select16(rowtop + b * RPB + c * RPC, dist + (b * RPB + c * RPC) * WH, WH, RPC, pg,
-                                        act, lane, stage_key + wave * 64 + grp * 16,
-                                        stage_xy + wave * 64 + grp * 16, key, kxy, thr);
-            if (act && staged <= 16 && sub < 8) {
-                fkey[u * 8 + sub] = key;
-                fxy[u * 8 + sub] = kxy;
-            }
-            if (act && sub == 0) {
-                over[u] = staged > 16 ? 1 : 0;
-                othr[u] = thr;
-            }
-        }
-        __syncthreads();
-        for (int u = wave; u < nbt * 2; u += 4) {
-            if (!over[u]) {
-                continue;
-            }
-            const int b = u >> 1, c = u & 1;
-            const PredictiveRows pg{geo + b * NW + c * PS, WH, win, inv_wh};
-            TopList L;
-            select_wave(rowtop + b * RPB + c * RPC, dist + (b * RPB + c * RPC) * WH, WH, RPC, pg, othr[u], lane,
-                        stage_key + wave * 64, stage_xy + wave * 64, L);
-            if (lane < 8) {
-                fkey[u * 8 + lane] = L.key;
-                fxy[u * 8 + lane] = L.xy;
-            }
-        }
-        __syncthreads();
-        // next predictors; the first ps_num entries kept for the group list
-        for (int i = tid; i < nbt * 16; i += 256) {
-            const int u = i >> 3, j = i & 7;
-            pred[i] = fxy[i];
-            if (j < PS) {
-                uint32_t *sv = fsave + ((u * 2 + (t - 1)) * PS + j) * 2;
-                sv[0] = uint32_t(fkey[i] >> 32);
-                sv[1] = fxy[i];
-            }
-        }
-        __syncthreads();
-    }
-
-    // group list: spatial top-8, then the temporal inserts in the shader's
-    // order, then the self-block rule; one thread per block
-    if (tid < nbt) {
-        const int blk = blk0 + tid;
-        float e[8];
-        uint32_t xy[8];
-        int z[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint2 sp = A.spatial[(long long)blk * 8 + j];
-            e[j] = __uint_as_float(sp.x);
-            xy[j] = sp.y;
-            z[j] = r;
-        }
-        for (int c = 0; c < 2; ++c) {
-            for (int t = 1; t <= r; ++t) {
-                const int zz = c == 0 ? r - t : r + t;
-                for (int i = 0; i < PS; ++i) {
-                    const uint32_t *sv = fsave + (((tid * 2 + c) * 2 + (t - 1)) * PS + i) * 2;
-                    const float fe = __uint_as_float(sv[0]);
-                    const uint32_t fp = sv[1];
-                    if (!(fe < e[7])) {
-                        continue;  // ginsert8: strict '<', after equal ones
-                    }
-                    int pos = 0;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        pos += e[j] <= fe ? 1 : 0;
-                    }
-#pragma unroll
-                    for (int j = 7; j > 0; --j) {
-                        if (j > pos) {
-                            e[j] = e[j - 1];
-                            xy[j] = xy[j - 1];
-                            z[j] = z[j - 1];
-                        }
-                    }
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        if (j == pos) {
-                            e[j] = fe;
-                            xy[j] = fp;
-                            z[j] = zz;
-                        }
-                    }
-                }
-            }
-        }
-        const int by = blk / A.nx, bx = blk - by * A.nx;
-        const int x = min(bx * A.step, W - 8), y = min(by * A.step, H - 8);
-        const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
-        bool present = false;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            present = present || (xy[j] == self_xy && z[j] == r);
-        }
-        if (!present) {
-#pragma unroll
-            for (int j = 7; j > 0; --j) {
-                e[j] = e[j - 1];
-                xy[j] = xy[j - 1];
-                z[j] = z[j - 1];
-            }
-            e[0] = -1.0f;
-            xy[0] = self_xy;
-            z[0] = r;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            A.groups[(long long)blk * 8 + j] = pack_entry(xy[j] & 0xFFFFu, xy[j] >> 16, uint32_t(z[j]));
-            if (A.dists) {
-                A.dists[(long long)blk * 8 + j] = e[j];
-            }
         }
     }
 }
@@ -1839,16 +1563,6 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
             hipLaunchKernelGGL((bm3d_match_rows<kFpExplicit>), rgrid, block, lds, s, rows);
         }
         if (a.radius == 0) {
-            return hipGetLastError();
-        }
-        if (a.t_blocks > 0) {
-            const size_t tlds = size_t(temporal_lds_bytes(a.t_blocks, a.ps_num, a.ps_range));
-            const dim3 tgrid((nref + a.t_blocks - 1) / a.t_blocks);
-            if (fused) {
-                hipLaunchKernelGGL((bm3d_match_temporal<kFpAll>), tgrid, block, tlds, s, a);
-            } else {
-                hipLaunchKernelGGL((bm3d_match_temporal<kFpExplicit>), tgrid, block, tlds, s, a);
-            }
             return hipGetLastError();
         }
         if (fused) {
