@@ -209,9 +209,13 @@ __device__ __forceinline__ void list_offer(TopList &L, bool cand, float d, int s
 template <int CM>
 __device__ __forceinline__ float col_ssd(const float *c, const float *s)
 {
-    float e0 = 0.0f, e1 = 0.0f;
+    // the shader's sums start at 0; 0 + v*v is v*v exactly (a square is
+    // never -0), so the first term initialises them
+    const float d0 = c[0] - s[0];
+    const float d1 = c[1] - s[1];
+    float e0 = d0 * d0, e1 = d1 * d1;
 #pragma unroll
-    for (int i = 0; i < 8; i += 2) {
+    for (int i = 2; i < 8; i += 2) {
         const float v0 = c[i] - s[i];
         const float v1 = c[i + 1] - s[i + 1];
         if (CM == kFpAll) {
