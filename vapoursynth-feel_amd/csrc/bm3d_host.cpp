@@ -163,6 +163,55 @@ struct Instance {
     GraphKey graph_key{};
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;  // orders the instance stream with a null-stream caller
 
+    // pinned host staging of the host-buffer path (allocated on first use):
+    // two upload planes used in turn, each with the event of its last copy,
+    // and one download plane; all with the library's plane pitch
+    float *pin_up[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    int pin_next = 0;
+    float *pin_down = nullptr;
+
+    hipError_t pin_alloc()
+    {
+        if (pin_down) {
+            return hipSuccess;
+        }
+        const size_t bytes = size_t(pe) * sizeof(float);
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = hipHostMalloc(reinterpret_cast<void **>(&pin_up[k]), bytes, hipHostMallocDefault);
+            if (e == hipSuccess) {
+                e = hipEventCreateWithFlags(&pin_ev[k], hipEventDisableTiming);
+            }
+        }
+        if (e == hipSuccess) {
+            e = hipHostMalloc(reinterpret_cast<void **>(&pin_down), bytes, hipHostMallocDefault);
+        }
+        return e;
+    }
+
+    // copies a host plane (row stride in bytes) to a device plane of the
+    // library's pitch: rows into a pinned plane, then one DMA copy
+    hipError_t upload(float *dev, const float *src, ptrdiff_t stride, hipStream_t s)
+    {
+        const int k = pin_next;
+        pin_next ^= 1;
+        hipError_t e = hipEventSynchronize(pin_ev[k]);  // its previous copy has left the buffer
+        if (e != hipSuccess) {
+            return e;
+        }
+        const size_t row_bytes = size_t(width) * sizeof(float);
+        for (int y = 0; y < height; ++y) {
+            std::memcpy(pin_up[k] + (size_t)y * pitch, reinterpret_cast<const char *>(src) + (ptrdiff_t)y * stride,
+                        row_bytes);
+        }
+        e = hipMemcpyAsync(dev, pin_up[k], size_t(pe) * sizeof(float), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            e = hipEventRecord(pin_ev[k], s);
+        }
+        return e;
+    }
+
     // optional per-kernel timing: HIP events recorded around every launch on
     // the launch stream, harvested on request
     struct Timed {
@@ -260,6 +309,17 @@ struct Instance {
         }
         if (graph_exec) {
             (void)hipGraphExecDestroy(graph_exec);
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (pin_up[k]) {
+                (void)hipHostFree(pin_up[k]);
+            }
+            if (pin_ev[k]) {
+                (void)hipEventDestroy(pin_ev[k]);
+            }
+        }
+        if (pin_down) {
+            (void)hipHostFree(pin_down);
         }
         if (fork_ev) {
             (void)hipEventDestroy(fork_ev);
@@ -793,6 +853,7 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
     std::lock_guard<std::mutex> lock(d.mu);
     VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
     hipStream_t s = d.stream;
+    VSFEEL_HIP(d.pin_alloc(), "hipHostMalloc");
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
     const int r = d.radius, nf = d.nframes;
     const int lo = clampi(n - 2 * r, 0, nf - 1), hi = clampi(n + 2 * r, 0, nf - 1);
@@ -811,14 +872,9 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
             return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing plane of frame " + std::to_string(f));
         }
         // in the final pass the slot holds [ref][source] like the reference's ring
-        VSFEEL_HIP(hipMemcpy2DAsync(d.src_slot(slot, d.clips - 1), size_t(d.pitch) * sizeof(float), clip_planes[i],
-                                    size_t(clip_strides[i]), row_bytes, size_t(d.height), hipMemcpyHostToDevice, s),
-                   "hipMemcpy2DAsync");
+        VSFEEL_HIP(d.upload(d.src_slot(slot, d.clips - 1), clip_planes[i], clip_strides[i], s), "upload");
         if (d.final_pass) {
-            VSFEEL_HIP(hipMemcpy2DAsync(d.src_slot(slot, 0), size_t(d.pitch) * sizeof(float), ref_planes[i],
-                                        size_t(ref_strides[i]), row_bytes, size_t(d.height), hipMemcpyHostToDevice,
-                                        s),
-                       "hipMemcpy2DAsync");
+            VSFEEL_HIP(d.upload(d.src_slot(slot, 0), ref_planes[i], ref_strides[i], s), "upload");
         }
         d.src_frame[slot] = f;
     }
@@ -844,11 +900,14 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         d.res_frame[slot] = r == 0 ? -1 : m;
     }
     VSFEEL_HIP(d.aggregate(n, slot_of, d.dst_dev, d.pitch, s), "aggregate launch");
-    VSFEEL_HIP(hipMemcpy2DAsync(dst, size_t(dst_stride), d.dst_dev, size_t(d.pitch) * sizeof(float), row_bytes,
-                                size_t(d.height), hipMemcpyDeviceToHost, s),
-               "hipMemcpy2DAsync");
+    VSFEEL_HIP(hipMemcpyAsync(d.pin_down, d.dst_dev, size_t(d.pe) * sizeof(float), hipMemcpyDeviceToHost, s),
+               "hipMemcpyAsync");
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
     VSFEEL_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    for (int y = 0; y < d.height; ++y) {
+        std::memcpy(reinterpret_cast<char *>(dst) + (ptrdiff_t)y * dst_stride, d.pin_down + (size_t)y * d.pitch,
+                    row_bytes);
+    }
     return VSFEEL_BM3D_OK;
 }
 
