@@ -22,10 +22,13 @@ forms (`forms`):
   radius 2, ps_num 2, ps_range 7, extractor_exp 2;
 * ``host_io``: the headline workload through the host-buffer entry point
   (the VapourSynth getFrame path): upload, filter and download per frame
-  request, so PCIe-inclusive.
+  request, so PCIe-inclusive;
+* ``c4``: 16 frames of 3840x2160 (radius 1, ps 2/7, sigma 5) in the FP32
+  form, with frame 0 compared against the FP64 oracle (max-abs, PSNR, share
+  of identical groups).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
-                    [--forms plain,fwd_bwd,temporal_r2,host_io] [--cpu-seconds S]
+                    [--forms plain,fwd_bwd,temporal_r2,host_io,c4] [--cpu-seconds S]
                     [--copy-calibration-mb MB]
 """
 
@@ -191,6 +194,40 @@ def bench_fwd_bwd(steps, warmup):
     return clip, kw, secs, per_kernel
 
 
+def bench_c4(steps, warmup, oracle_threads):
+    """The precision-form workload (16 frames of 3840x2160, shift (1, 0),
+    noise 5/255, sigma 5, step 4, range 16, radius 1, ps 2/7) in the FP32
+    form: frames/s, and on frame 0 the error against the FP64 oracle (max-abs,
+    PSNR) and the share of its groups identical to the FP64 oracle's."""
+    import torch
+    from oracle import pyoracle
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+    kw = dict(synth.WORKLOADS[4]["args"])
+    clip = synth.workload(4)
+    nf, h, w = clip.shape
+    x = torch.from_numpy(clip).to("cuda:0")
+    inst = Instance(nf, h, w, **kw)
+    out = torch.empty_like(x)
+    secs, per_kernel = timed(inst, lambda: inst.process(x, out=out), steps, warmup)
+    got = out[0].cpu().numpy().astype(np.float64)
+    g32, _ = inst.groups(x, 0)
+    p = pyoracle.make_params(w, h, nthreads=oracle_threads, **kw)
+    ref = pyoracle.clip(p, clip, first=0, count=1, f64=True)[0]
+    r = kw.get("radius", 0)
+    window = [clip[min(max(0 - r + z, 0), nf - 1)] for z in range(2 * r + 1)]
+    _, g64 = pyoracle.estimate(p, window, f64=True)
+    same = (g32 == g64).all(axis=(2, 3))
+    mse = float(np.mean((got - ref) ** 2))
+    return dict(value=nf * steps / secs, unit="frames/s", ms_per_step=secs * 1e3 / steps,
+                workload=describe(4, clip.shape, kw, "plain FP32 form"),
+                kernels_ms_avg={k: v[0] for k, v in per_kernel.items()},
+                vs_fp64_oracle=dict(frame=0, max_abs=float(np.abs(got - ref).max()),
+                                    psnr_db=10.0 * np.log10(1.0 / mse) if mse > 0 else None,
+                                    groups_identical=float(same.mean()),
+                                    oracle_threads=oracle_threads))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,7 +235,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--frames", type=int, default=None, help="override the clip length")
-    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2,host_io",
+    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2,host_io,c4",
                     help="comma list of forms to time (plain is the headline value)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-time budget of the CPU baseline sample (0: skip)")
@@ -241,6 +278,8 @@ def main():
                                                   "host buffers: upload, filter and download per "
                                                   "frame request, one thread (PCIe-inclusive)"))
         del node
+    if "c4" in forms:
+        extra["c4"] = bench_c4(max(1, args.steps // 2), 1, max(1, min(16, os.cpu_count() or 1)))
     if "temporal_r2" in forms:
         c3, kw3, s3, pk3 = bench_plain(3, max(1, args.steps // 2), 1, args.frames)
         extra["temporal_r2"] = dict(value=c3.shape[0] * max(1, args.steps // 2) / s3, unit="frames/s",
