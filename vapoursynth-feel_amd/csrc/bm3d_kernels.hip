@@ -879,12 +879,17 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
     __syncthreads();
 
     // phase 2: top-8 of each block by one 16-lane row of a wavefront (4
-    // blocks per wavefront at once, all cross-lane work inside DPP rows)
+    // blocks per wavefront at once, all cross-lane work inside DPP rows);
+    // blocks fill the rows of one wavefront before the next, and wavefronts
+    // without a block skip the pass (the kernel is bound by VALU issue)
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     for (int b0 = 0; b0 < nb; b0 += 16) {
+        if (b0 + 4 * wave >= nb) {
+            continue;  // whole wavefront
+        }
         const int grp = lane >> 4, sub = lane & 15;
-        const int b = b0 + wave + 4 * grp;
+        const int b = b0 + 4 * wave + grp;
         const bool act = b < nb;
         const int bs = act ? b : 0;
         const int bx = bx0 + bs;
