@@ -575,11 +575,11 @@ __device__ __forceinline__ float4 scan_row_sliding(const float *cb, const float 
         const bool c1 = dv < m1, c2 = dv < m2;
         k2 = c1 ? k1 : (c2 ? k : k2);
         k1 = c1 ? k : k1;
-        const float t1 = fmaxf(m1, dv);
+        // insertion into m1 <= m2 <= m3: the new third is the median of
+        // (m2, m3, dv), the new second that of (m1, m2, dv)
+        m3 = __builtin_amdgcn_fmed3f(m2, m3, dv);
+        m2 = __builtin_amdgcn_fmed3f(m1, m2, dv);
         m1 = fminf(m1, dv);
-        const float t2 = fmaxf(m2, t1);
-        m2 = fminf(m2, t1);
-        m3 = fminf(m3, t2);
     }
     return make_float4(m1, m2, m3, __uint_as_float(k1 | (k2 << 16)));
 }
