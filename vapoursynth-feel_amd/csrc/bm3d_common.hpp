@@ -35,7 +35,18 @@ struct MatchArgs {
     // layout (column-major source region, per-block distance rows)
     int run_blocks, runs_x;
     int region_cols, region_rows, dist_cols;
+    // temporal search after bm3d_match_rows: distance rows per wavefront
+    // (ps_num * (2 ps_range + 1)) for the threshold selection, 0: list
+    int t_rows;
 };
+
+// Dynamic LDS of bm3d_match with the temporal threshold selection: per
+// wavefront (4) a 64-entry stage (8-byte keys, 4-byte positions) and t_rows
+// distance rows of 2 ps_range + 1 floats.
+__host__ __device__ inline long long temporal_lds_bytes(int t_rows, int ps_range)
+{
+    return t_rows > 0 ? 4LL * (64 * 12 + 4LL * t_rows * (2LL * ps_range + 1)) : 0;
+}
 
 // LDS bytes of bm3d_match_rows: survivor stages of its 4 wavefronts (64 keys
 // + positions each), per window row its 3 smallest distances and the

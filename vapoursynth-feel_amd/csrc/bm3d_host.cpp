@@ -128,6 +128,7 @@ struct Instance {
     // workgroup run and the LDS layout; run_blocks == 0 selects the
     // per-wavefront search instead
     int run_blocks = 0, runs_x = 0, region_cols = 0, region_rows = 0, dist_cols = 0;
+    int t_rows = 0;  // temporal threshold selection: distance rows per wavefront (0: list)
     float *dst_dev = nullptr;    // one plane
 
     // tiled accumulation of the frame's own slice (see bm3d_estimate_tiled)
@@ -367,6 +368,7 @@ struct Instance {
         m.region_cols = region_cols;
         m.region_rows = region_rows;
         m.dist_cols = dist_cols;
+        m.t_rows = t_rows;
         return m;
     }
 
@@ -671,6 +673,14 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
                 d.region_cols = cols;
                 d.region_rows = rh + 7;
                 d.dist_cols = dcols;
+            }
+        }
+        if (d.run_blocks > 0 && d.radius > 0) {
+            // the predictive search's rows of one step, one per lane
+            const int wh = 2 * d.ps_range + 1;
+            const int rows = d.ps_num * wh;
+            if (rows <= 64 && temporal_lds_bytes(rows, d.ps_range) <= 48 * 1024) {
+                d.t_rows = rows;
             }
         }
     }

@@ -68,13 +68,6 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
     return (uint64_t(hi) << 32) | lo;
 }
 
-__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d)
-{
-    const uint32_t lo = uint32_t(__shfl_up(int(uint32_t(v)), d));
-    const uint32_t hi = uint32_t(__shfl_up(int(uint32_t(v >> 32)), d));
-    return (uint64_t(hi) << 32) | lo;
-}
-
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l)
 {
     return uint32_t(__builtin_amdgcn_readlane(int(v), l));
@@ -236,12 +229,14 @@ __device__ __forceinline__ float col_ssd(const float *c, const float *s)
 // column values of lanes k..k+7 -- exactly what the shader's shifted column
 // sums add up to.  Short rows share the wave (several row segments per pass);
 // long rows are cut into chunks of 57 candidates.  Each lane walks down its
-// column with an 8-row register window.
+// column with an 8-row register window.  The candidates go to the list L,
+// or (TO_ROWS) their distances to drows[row * drow_stride + k].
 // ---------------------------------------------------------------------------
-template <int CM, bool EXACT>
+template <int CM, bool EXACT, bool TO_ROWS = false>
 __device__ void scan_window(TopList &L, const float *__restrict__ plane, int pitch, int height,
                             const float *cur_lds, const float *cur_reg, int left, int right,
-                            int top, int bottom, int seq_base, int lane, int skip_row = -1)
+                            int top, int bottom, int seq_base, int lane, int skip_row = -1,
+                            float *drows = nullptr, int drow_stride = 0)
 {
     const int rw = right - left + 1;
     const int rh = bottom - top + 1;
@@ -301,9 +296,15 @@ __device__ void scan_window(TopList &L, const float *__restrict__ plane, int pit
             }
             const int row = cy - top;
             const bool cand = lane_on && kk < nc && row < rh && cy != skip_row;
-            const int seq = seq_base + row * rw + j;
-            const uint32_t xy = uint32_t(left + j) | (uint32_t(cy) << 16);
-            list_offer(L, cand, d, seq, xy, lane);
+            if (TO_ROWS) {
+                if (cand) {
+                    drows[row * drow_stride + j] = d;
+                }
+            } else {
+                const int seq = seq_base + row * rw + j;
+                const uint32_t xy = uint32_t(left + j) | (uint32_t(cy) << 16);
+                list_offer(L, cand, d, seq, xy, lane);
+            }
         }
     }
 }
@@ -414,25 +415,6 @@ __device__ __forceinline__ float wave_min(float v)
     const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
     const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
     return fminf(fminf(r0, r1), fminf(r2, r3));
-}
-
-// 8th smallest of the 64 lane values, ties counted; FLT_MAX when fewer than
-// 8 lanes hold a value below FLT_MAX
-__device__ __forceinline__ float kth_smallest8(float a, int lane)
-{
-    (void)lane;
-    int taken = 0;
-    float m = FLT_MAX;
-    for (int it = 0; it < 8 && taken < 8; ++it) {
-        m = wave_min(a);
-        if (!(m < FLT_MAX)) {
-            return FLT_MAX;
-        }
-        const bool hit = a == m;
-        taken += __popcll(__ballot(hit));
-        a = hit ? INFINITY : a;
-    }
-    return m;
 }
 
 // lane value of the partner lane ^ J inside a 16-lane row
@@ -787,6 +769,127 @@ __device__ void select_wave(const float4 *rt, const float *db, int dcols, int nr
 }
 
 // ---------------------------------------------------------------------------
+// Frame top-8 of one temporal step (src/bm3d.comp:569-624) from the distance
+// rows of its ps_num predictor windows: window w's row r at row slot
+// w * wh + r of tdist (row stride wh), predictor w's position in lane w of
+// pred.  One lane per row slot keeps the row's 3 smallest distances and
+// the positions of the first two; the 8th smallest of those (ties counted)
+// bounds the step's 8th smallest distance from above; the candidates at or
+// below it are gathered as in select16 (a row whose third smallest is above
+// it holds them among its stored first two, other rows are scanned), staged
+// (64 at a time, merged into the list when more) and sorted.  seq of window
+// w's row r, position k is w * wh^2 + r * rw_w + k.  Needs ps_num * wh <= 64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ TopList select_predictive(const float *tdist, uint32_t pred, int ps_num, int ps_range,
+                                                     int W, int H, unsigned long long *sk, uint32_t *sxy, int lane)
+{
+    const int wh = 2 * ps_range + 1, win = wh * wh, nrows = ps_num * wh;
+    const bool slot = lane < nrows;
+    const int w = slot ? div_small(lane, 1.0f / float(wh)) : 0;
+    const int r = lane - w * wh;
+    const uint32_t pxy = uint32_t(__shfl(int(pred), w));
+    const int cx = int(pxy & 0xFFFFu), cy = int(pxy >> 16);
+    const int left = max(cx - ps_range, 0), top = max(cy - ps_range, 0);
+    const int rw = min(cx + ps_range, W - 8) - left + 1;
+    const int rh = min(cy + ps_range, H - 8) - top + 1;
+    const bool row_ok = slot && r < rh;
+    float m1 = FLT_MAX, m2 = FLT_MAX, m3 = FLT_MAX;
+    uint32_t k1 = 0, k2 = 0;
+    if (row_ok) {
+        const float *dr = tdist + lane * wh;
+        for (int k = 0; k < rw; ++k) {
+            const float d = dr[k];
+            const float dv = (d < FLT_MAX) ? d : FLT_MAX;  // FLT_MAX and NaN never count
+            const bool c1 = dv < m1, c2 = dv < m2;
+            k2 = c1 ? k1 : (c2 ? uint32_t(k) : k2);
+            k1 = c1 ? uint32_t(k) : k1;
+            m3 = __builtin_amdgcn_fmed3f(m2, m3, dv);
+            m2 = __builtin_amdgcn_fmed3f(m1, m2, dv);
+            m1 = fminf(m1, dv);
+        }
+    }
+    // threshold: 8 rounds of the wave minimum of the lanes' smallest
+    // remaining stored value
+    float h0 = m1, h1 = m2, h2 = m3;
+    float thr = FLT_MAX;
+    int taken = 0;
+    for (int it = 0; it < 8; ++it) {
+        const float m = wave_min(h0);
+        if (!(m < FLT_MAX)) {
+            thr = FLT_MAX;  // fewer than 8 candidates: every valid one survives
+            break;
+        }
+        const bool hit = h0 == m;
+        taken += __popcll(__ballot(hit));
+        thr = m;
+        if (taken >= 8) {
+            break;
+        }
+        if (hit) {
+            h0 = h1;
+            h1 = h2;
+            h2 = FLT_MAX;
+        }
+    }
+    TopList L;
+    list_reset(L);
+    int staged = 0;
+    bool merged = false;
+    auto push = [&](bool s, float d, int seq, uint32_t xy) {
+        const uint64_t m = __ballot(s);
+        const int cnt = __popcll(m);
+        if (staged + cnt > 64) {
+            wave_lds_sync();
+            const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+            const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+            wave_lds_sync();
+            list_merge_batch(L, key, kxy, lane);
+            staged = 0;
+            merged = true;
+        }
+        if (s) {
+            const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
+            sk[p] = make_key(d, seq);
+            sxy[p] = xy;
+        }
+        staged += cnt;
+    };
+    // rows holding 3 or more candidates at or below thr: scanned
+    const bool scan = row_ok && m3 <= thr && m3 < FLT_MAX;
+    uint64_t rows = __ballot(scan);
+    while (rows) {
+        const int rl = __ffsll((unsigned long long)rows) - 1;
+        rows &= rows - 1;
+        const int sw = __builtin_amdgcn_readlane(w, rl), sr = __builtin_amdgcn_readlane(r, rl);
+        const int sl = __builtin_amdgcn_readlane(left, rl), st = __builtin_amdgcn_readlane(top, rl);
+        const int srw = __builtin_amdgcn_readlane(rw, rl);
+        const float d = lane < srw ? tdist[rl * wh + lane] : FLT_MAX;  // srw <= wh <= 64
+        const bool surv = lane < srw && d <= thr && d < FLT_MAX;
+        push(surv, d, sw * win + sr * srw + lane, uint32_t(sl + lane) | (uint32_t(st + sr) << 16));
+    }
+    // the other rows: their stored first two
+    const bool s0 = row_ok && !scan && m1 <= thr && m1 < FLT_MAX;
+    push(s0, m1, w * win + r * rw + int(k1), uint32_t(left + int(k1)) | (uint32_t(top + r) << 16));
+    const bool s1 = row_ok && !scan && m2 <= thr && m2 < FLT_MAX;
+    push(s1, m2, w * win + r * rw + int(k2), uint32_t(left + int(k2)) | (uint32_t(top + r) << 16));
+    wave_lds_sync();
+    uint64_t key = lane < staged ? sk[lane] : kNoKey;
+    uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+    wave_lds_sync();
+    if (!merged && staged <= 16) {
+        // the usual case: the few survivors sorted inside one DPP row
+        sort16_dpp(key, kxy, lane);
+        if (lane < 8) {
+            L.key = key < kPlaceholderKey ? key : kPlaceholderKey;
+            L.xy = key < kPlaceholderKey ? kxy : 0u;
+        }
+    } else {
+        list_merge_batch(L, key, kxy, lane);
+    }
+    return L;
+}
+
+// ---------------------------------------------------------------------------
 // Spatial search of the sliding metric, row-parallel.  A workgroup takes a
 // run of run_blocks consecutive ref blocks of one grid row; their windows
 // share the rows [top, bottom], so one source region (column-major in LDS)
@@ -948,6 +1051,9 @@ template <int CM, bool EXACT>
 __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
 {
     __shared__ float cur_all[4][64];
+    // temporal selection (t_rows > 0): per wavefront a 64-entry stage (keys,
+    // positions) and t_rows distance rows of 2 ps_range + 1 floats
+    extern __shared__ __align__(16) unsigned long long tlds[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
 
@@ -1014,13 +1120,33 @@ __global__ __launch_bounds__(256) void bm3d_match(MatchArgs A)
                 const float *plane = A.plane[z];
                 TopList F;
                 list_reset(F);
-                for (int i = 0; i < A.ps_num; ++i) {
-                    const uint32_t pxy = readlane_u32(pred, i);
-                    const int cx = int(pxy & 0xFFFFu), cyy = int(pxy >> 16);
-                    const int left = max(cx - PR, 0), right = min(cx + PR, W - 8);
-                    const int top = max(cyy - PR, 0), bottom = min(cyy + PR, H - 8);
-                    scan_window<CM, EXACT>(F, plane, pitch, H, cur_lds, cur_reg, left, right, top, bottom,
-                                           i * win, lane);
+                if (!EXACT && A.t_rows > 0) {
+                    // distance rows of the ps_num windows in this wavefront's
+                    // LDS, then one threshold selection
+                    const int wh = 2 * PR + 1;
+                    unsigned long long *sk = tlds + wave * 64;
+                    uint32_t *sxy = reinterpret_cast<uint32_t *>(tlds + 256) + wave * 64;
+                    float *tdist = reinterpret_cast<float *>(tlds + 256 + 128) + wave * A.t_rows * wh;
+                    for (int i = 0; i < A.ps_num; ++i) {
+                        const uint32_t pxy = readlane_u32(pred, i);
+                        const int cx = int(pxy & 0xFFFFu), cyy = int(pxy >> 16);
+                        const int left = max(cx - PR, 0), right = min(cx + PR, W - 8);
+                        const int top = max(cyy - PR, 0), bottom = min(cyy + PR, H - 8);
+                        scan_window<CM, false, true>(F, plane, pitch, H, cur_lds, cur_reg, left, right, top,
+                                                     bottom, 0, lane, -1, tdist + i * wh * wh, wh);
+                    }
+                    wave_lds_sync();
+                    F = select_predictive(tdist, pred, A.ps_num, PR, W, H, sk, sxy, lane);
+                    wave_lds_sync();
+                } else {
+                    for (int i = 0; i < A.ps_num; ++i) {
+                        const uint32_t pxy = readlane_u32(pred, i);
+                        const int cx = int(pxy & 0xFFFFu), cyy = int(pxy >> 16);
+                        const int left = max(cx - PR, 0), right = min(cx + PR, W - 8);
+                        const int top = max(cyy - PR, 0), bottom = min(cyy + PR, H - 8);
+                        scan_window<CM, EXACT>(F, plane, pitch, H, cur_lds, cur_reg, left, right, top, bottom,
+                                               i * win, lane);
+                    }
                 }
                 for (int i = 0; i < A.ps_num; ++i) {
                     const uint64_t fk = readlane_u64(F.key, i);
@@ -1574,10 +1700,11 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
         if (a.radius == 0) {
             return hipGetLastError();
         }
+        const size_t tlds = size_t(temporal_lds_bytes(a.t_rows, a.ps_range));
         if (fused) {
-            hipLaunchKernelGGL((bm3d_match<kFpAll, false>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((bm3d_match<kFpAll, false>), grid, block, tlds, s, a);
         } else {
-            hipLaunchKernelGGL((bm3d_match<kFpExplicit, false>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((bm3d_match<kFpExplicit, false>), grid, block, tlds, s, a);
         }
         return hipGetLastError();
     }

@@ -64,7 +64,8 @@ class Instance:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        lib = _lib._lib
+        # at interpreter exit the module globals may already be gone
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
         if h and lib is not None:
             lib.vsfeel_bm3d_free(h)
         self.handle = None
