@@ -1483,26 +1483,20 @@ __global__ __launch_bounds__(256) void bm3d_estimate(EstimateArgs A)
     }
 }
 
-// Tiled estimate: a workgroup filters the groups of a tile of tile_bx x
-// tile_by reference blocks.  Their blocks in the reference block's own frame
-// lie within bm_range of it, so those contributions are summed in an LDS
-// accumulator covering the tile plus that margin, then written out
-// (non-atomically) to the tile's slot of the scratch buffer; the
-// bm3d_tile_reduce pass adds the overlapping tiles in a fixed order.  Blocks
-// in the other frames of the temporal window go to the dense slices with
-// global atomics.
-// Q31.32 fixed point of v, rounded toward zero: exact for every value the
-// extractor produces (multiples of 2^-22), and integer sums do not depend on
-// the order of the additions.
+// Fixed point of v with 31 fraction bits, as a Q31.32 integer (the last bit
+// is 0), rounded toward zero: exact for every value the extractor produces
+// (multiples of 2^-22), and integer sums do not depend on the order of the
+// additions.  v = hi + lo with hi = trunc(v) and |lo| < 1, both exact, so
+// the integer is hi * 2^32 + trunc(lo * 2^31) * 2 (|v| < 2^31).
 __device__ __forceinline__ long long to_fixed(float v)
 {
-    const uint32_t b = __float_as_uint(v);
-    const int e = int((b >> 23) & 0xFFu);
-    const uint64_t m = uint64_t((b & 0x7FFFFFu) | 0x800000u);
-    const int sh = e - 118;  // v * 2^32 = m * 2^(e - 150 + 32)
-    uint64_t mag = sh >= 0 ? (m << min(sh, 39)) : (m >> min(-sh, 63));
-    mag = e == 0 ? 0 : mag;  // zero and subnormals
-    return (b >> 31) ? -(long long)mag : (long long)mag;
+    const float hi = __builtin_truncf(v);
+    const float lo = v - hi;
+    const int hi_i = int(hi);
+    const int lo_i = int(lo * 2147483648.0f);
+    const uint32_t word_hi = uint32_t(hi_i + (lo_i >> 31));
+    const uint32_t word_lo = uint32_t(lo_i) << 1;
+    return (long long)((uint64_t(word_hi) << 32) | word_lo);
 }
 
 constexpr int kTiledThreads = 512;
