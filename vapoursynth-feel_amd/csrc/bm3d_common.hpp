@@ -85,6 +85,20 @@ struct EstimateArgs {
     // (kFilterApply)
     uint64_t *masks;
     float *weights;
+    // radius > 0: the slices of the other frames receive each block's weight
+    // once, at its origin, in these planes ([2r][height][origin_pitch], the
+    // reference frame's slice skipped); bm3d_weight_box expands them
+    float *origins;
+    int origin_pitch;
+};
+
+// 8x8 box sums of the origin weight planes into the weight planes of the
+// other frames' slices of one estimate stack.
+struct BoxArgs {
+    const float *origins;            // [2r][height][origin_pitch]
+    float *res;                      // the estimate stack, [2r+1][2][height][res_pitch]
+    int width, height, origin_pitch, res_pitch;
+    int radius;
 };
 
 enum FilterMode : int {

@@ -27,6 +27,7 @@ hipError_t launch_divide(const float *num, int num_pitch, const float *den, int 
 hipError_t launch_aggregate(const AggArgs &a, hipStream_t s);
 hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
 hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s);
+hipError_t launch_weight_box(const BoxArgs &a, hipStream_t s);
 
 namespace {
 
@@ -59,7 +60,15 @@ int agg_z(int i, int n, int nframes, int radius)
     return std::min(std::max(2 * radius - i, n - nframes + 1 + radius), n + radius);
 }
 
-enum KernelKind { kKernelFill = 0, kKernelMatch, kKernelEstimate, kKernelAggregate, kKernelReduce, kKernelKinds };
+enum KernelKind {
+    kKernelFill = 0,
+    kKernelMatch,
+    kKernelEstimate,
+    kKernelAggregate,
+    kKernelReduce,
+    kKernelWeightBox,
+    kKernelKinds
+};
 
 // LDS bytes a tile accumulator may take (one 512-thread workgroup per CU,
 // 16 bytes per region pixel), and the groups one pass of it filters
@@ -136,6 +145,7 @@ struct Instance {
     bool short_windows = false;  // some spatial window has fewer than 8 candidates
     int tile_bx = 0, tile_by = 0, tiles_x = 0, tiles_y = 0, region_w = 0, region_h = 0;
     float *scratch = nullptr;    // tiles * 2 * region_h * region_w
+    float *origins = nullptr;    // radius > 0: 2r origin weight planes of height * pitch
     int *tile_tables = nullptr;  // col_tiles[W] (int2), row_tiles[H] (int2), tile_x0[tiles_x], tile_y0[tiles_y]
     hipStream_t stream = nullptr;
     hipEvent_t last_done = nullptr;  // end of the previous device-side call
@@ -304,6 +314,7 @@ struct Instance {
         (void)hipFree(spatial);
         (void)hipFree(dst_dev);
         (void)hipFree(scratch);
+        (void)hipFree(origins);
         (void)hipFree(tile_tables);
         if (last_done) {
             (void)hipEventDestroy(last_done);
@@ -392,6 +403,14 @@ struct Instance {
                 return e;
             }
         }
+        if (origins) {
+            e = timed_launch(kKernelFill, s, [&] {
+                return launch_fill_zero(origins, 2LL * radius * height * pitch, s);
+            });
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
         uint32_t *table = saved.groups ? saved.groups : groups;
         if (mode != kFilterApply) {
             MatchArgs m = match_args(search, in_pitch, dists_out);
@@ -422,6 +441,8 @@ struct Instance {
         a.ny = ny;
         a.step = block_step;
         a.halo = bm_range;
+        a.origins = origins;
+        a.origin_pitch = pitch;
         if (tiled) {
             a.scratch = scratch;
             a.tile_bx = tile_bx;
@@ -432,6 +453,17 @@ struct Instance {
             a.region_h = region_h;
         }
         e = timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, mode, s); });
+        if (e == hipSuccess && origins) {
+            BoxArgs b {};
+            b.origins = origins;
+            b.res = stack;
+            b.width = width;
+            b.height = height;
+            b.origin_pitch = pitch;
+            b.res_pitch = stack_pitch;
+            b.radius = radius;
+            e = timed_launch(kKernelWeightBox, s, [&] { return launch_weight_box(b, s); });
+        }
         if (e != hipSuccess || !tiled) {
             return e;
         }
@@ -752,6 +784,7 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         (e = hipMalloc(&d.groups, size_t(d.nref) * 8 * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMalloc(&d.dists, size_t(d.nref) * 8 * sizeof(float))) != hipSuccess ||
         (d.radius > 0 && (e = hipMalloc(&d.spatial, size_t(d.nref) * 8 * sizeof(uint2))) != hipSuccess) ||
+        (d.radius > 0 && (e = hipMalloc(&d.origins, plane * 2 * d.radius)) != hipSuccess) ||
         (e = hipMalloc(&d.dst_dev, plane)) != hipSuccess) {
         return dev_fail(e, "hipMalloc");
     }

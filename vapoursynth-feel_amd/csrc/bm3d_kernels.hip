@@ -1441,12 +1441,24 @@ __device__ __forceinline__ void scatter_global(const EstimateArgs &A, uint32_t e
 {
     const long long rp = A.res_pitch;
     const long long pe = (long long)A.height * rp;
-    float *wd = A.res + (long long)entry_z(e) * 2 * pe + (long long)entry_y(e) * rp + entry_x(e) + c;
+    const int z = int(entry_z(e));
+    float *wd = A.res + (long long)z * 2 * pe + (long long)entry_y(e) * rp + entry_x(e) + c;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const float v = fma_c<CM>(w, blk[j], A.extractor) - A.extractor;
         unsafeAtomicAdd(wd + j * rp, v);
-        unsafeAtomicAdd(wd + pe + j * rp, wq);
+    }
+    if (A.origins != nullptr && z != A.radius) {
+        // another frame's slice: the weight once, at the block's origin
+        if (c == 0) {
+            const int k = z < A.radius ? z : z - 1;
+            unsafeAtomicAdd(A.origins + ((long long)k * A.height + entry_y(e)) * A.origin_pitch + entry_x(e), wq);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            unsafeAtomicAdd(wd + pe + j * rp, wq);
+        }
     }
 }
 
@@ -1624,6 +1636,56 @@ __global__ __launch_bounds__(256) void bm3d_tile_reduce(ReduceArgs A)
     A.res[qw] = wt;
 }
 
+// Weight planes of the other frames' slices from their origin planes: pixel
+// (x, y) gets the sum of the origin weights at (x - 0..7, y - 0..7), i.e. the
+// weights of the blocks that cover it, summed in double and rounded once (the
+// exact sum wherever float atomics on every pixel would give it exactly).
+// A workgroup makes a 64 x 16 tile of one plane from a 71 x 23 LDS tile of
+// origins: sums along rows, then along columns.
+constexpr int kBoxW = 64, kBoxH = 16;
+__global__ __launch_bounds__(256) void bm3d_weight_box(BoxArgs A)
+{
+    __shared__ float o[kBoxH + 7][kBoxW + 8];
+    __shared__ double hs[kBoxH + 7][kBoxW];
+    const int k = blockIdx.z;
+    const int z = k < A.radius ? k : k + 1;
+    const float *src = A.origins + (long long)k * A.height * A.origin_pitch;
+    const long long pe = (long long)A.height * A.res_pitch;
+    float *dst = A.res + (long long)z * 2 * pe + pe;
+    const int x0 = blockIdx.x * kBoxW, y0 = blockIdx.y * kBoxH;
+    for (int i = threadIdx.x; i < (kBoxH + 7) * (kBoxW + 7); i += 256) {
+        const int r = i / (kBoxW + 7), cx = i % (kBoxW + 7);
+        const int gx = x0 - 7 + cx, gy = y0 - 7 + r;
+        o[r][cx] = (gx >= 0 && gx < A.width && gy >= 0 && gy < A.height)
+                       ? src[(long long)gy * A.origin_pitch + gx]
+                       : 0.0f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (kBoxH + 7) * kBoxW; i += 256) {
+        const int r = i / kBoxW, x = i % kBoxW;
+        double s = 0.0;
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) {
+            s += (double)o[r][x + dx];
+        }
+        hs[r][x] = s;
+    }
+    __syncthreads();
+    const int x = threadIdx.x & 63;
+    const int gx = x0 + x;
+    for (int y = threadIdx.x >> 6; y < kBoxH; y += 4) {
+        double s = 0.0;
+#pragma unroll
+        for (int dy = 0; dy < 8; ++dy) {
+            s += hs[y + dy][x];
+        }
+        const int gy = y0 + y;
+        if (gx < A.width && gy < A.height) {
+            dst[(long long)gy * A.res_pitch + gx] = (float)s;
+        }
+    }
+}
+
 // dst = float(sum double(wdst) / sum double(weight)) over the tw slices
 // (src/bm3d_agg.comp:36-51)
 __global__ __launch_bounds__(256) void bm3d_aggregate(AggArgs A)
@@ -1765,6 +1827,13 @@ hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s)
 {
     const dim3 grid((a.width + 63) / 64, (a.height + 3) / 4), block(256);
     hipLaunchKernelGGL(bm3d_tile_reduce, grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_weight_box(const BoxArgs &a, hipStream_t s)
+{
+    const dim3 grid((a.width + kBoxW - 1) / kBoxW, (a.height + kBoxH - 1) / kBoxH, 2 * a.radius), block(256);
+    hipLaunchKernelGGL(bm3d_weight_box, grid, block, 0, s, a);
     return hipGetLastError();
 }
 

@@ -34,7 +34,8 @@ EXPORTS = (
     "vsfeel_bm3d_set_graphs",
 )
 
-KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce")
+KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce",
+                "bm3d_weight_box")
 
 
 class Format(ctypes.Structure):
