@@ -434,13 +434,13 @@ __device__ __forceinline__ uint32_t dpp_xor(uint32_t v, int lane)
     }
 }
 
-// minimum over the 16 lanes of each row, in every lane of the row
-__device__ __forceinline__ float row_min16(float v, int lane)
+// minimum of an unsigned value over the 16 lanes of each row
+__device__ __forceinline__ uint32_t row_min16_u32(uint32_t v, int lane)
 {
-    v = fminf(v, __uint_as_float(dpp_xor<1>(__float_as_uint(v), lane)));
-    v = fminf(v, __uint_as_float(dpp_xor<2>(__float_as_uint(v), lane)));
-    v = fminf(v, __uint_as_float(dpp_xor<4>(__float_as_uint(v), lane)));
-    v = fminf(v, __uint_as_float(dpp_xor<8>(__float_as_uint(v), lane)));
+    v = min(v, dpp_xor<1>(v, lane));
+    v = min(v, dpp_xor<2>(v, lane));
+    v = min(v, dpp_xor<4>(v, lane));
+    v = min(v, dpp_xor<8>(v, lane));
     return v;
 }
 
@@ -485,25 +485,47 @@ __device__ __forceinline__ void sort16_dpp(uint64_t &key, uint32_t &xy, int lane
     sort16_step<16, 1>(key, xy, lane);
 }
 
+// Row keys of the spatial selection: the bits of a distance (a non-negative
+// float, so its bits order as the distance does) with the low bits replaced
+// by the candidate's position in its row (pm = the position mask).  Keys of
+// one row are distinct; a key bounds its distance's bits from below by
+// key & ~pm and from above by key | pm.
+__device__ __forceinline__ uint32_t row_key(float d, uint32_t k, uint32_t pm)
+{
+    return (__float_as_uint(d) & ~pm) | k;
+}
+
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c)
+{
+    return max(min(a, b), min(max(a, b), c));
+}
+
+// position mask for rows of up to n candidates
+__device__ __forceinline__ uint32_t pos_mask(int n)
+{
+    return n <= 2 ? 1u : (2u << (31 - __clz(n - 1))) - 1u;
+}
+
+constexpr uint32_t kFltMaxBits = 0x7F7FFFFFu;  // distances at or above FLT_MAX never count
+
 // ---------------------------------------------------------------------------
 // One window row of the sliding metric from a column-major LDS region (column
 // j of the row starts at col0 + j * CS): column SSDs v_j against reference
 // column min(j, 7) of cb, then
 // D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
-// written to drow[0 .. rw-1].  Returns the row's 3 smallest distances
-// (x <= y <= z; distances at or above FLT_MAX, and NaN, count as FLT_MAX)
-// and in w the positions of the first two (k1 | k2 << 16).
+// written to drow[0 .. rw-1].  Returns the row's 3 smallest keys (row_key
+// with mask pm; x <= y <= z, ~0 where the row has fewer candidates).
 // ---------------------------------------------------------------------------
 template <int CM>
-__device__ __forceinline__ float4 scan_row_sliding(const float *cb, const float *col0, int CS, int rw, float *drow)
+__device__ __forceinline__ uint4 scan_row_sliding(const float *cb, const float *col0, int CS, int rw, float *drow,
+                                                  uint32_t pm)
 {
     float c7[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         c7[i] = cb[56 + i];
     }
-    float m1 = FLT_MAX, m2 = FLT_MAX, m3 = FLT_MAX;
-    uint32_t k1 = 0, k2 = 0;
+    uint32_t m1 = ~0u, m2 = ~0u, m3 = ~0u;
     // Column by column with the partial sums of the tree kept rolling:
     // p1_i = v_i + v_i+1, p2_i = p1_i + p1_i+2, D_k = p2_k + p2_k+4.  Column j
     // completes p1_j-1, p2_j-3 and, from j = 7 on, D_j-7.
@@ -552,18 +574,14 @@ __device__ __forceinline__ float4 scan_row_sliding(const float *cb, const float 
         p2_4 = p2;
         const uint32_t k = uint32_t(j - 7);
         drow[k] = d;
-        // FLT_MAX and NaN never count (the shader's strict '<')
-        const float dv = (d < FLT_MAX) ? d : FLT_MAX;
-        const bool c1 = dv < m1, c2 = dv < m2;
-        k2 = c1 ? k1 : (c2 ? k : k2);
-        k1 = c1 ? k : k1;
         // insertion into m1 <= m2 <= m3: the new third is the median of
-        // (m2, m3, dv), the new second that of (m1, m2, dv)
-        m3 = __builtin_amdgcn_fmed3f(m2, m3, dv);
-        m2 = __builtin_amdgcn_fmed3f(m1, m2, dv);
-        m1 = fminf(m1, dv);
+        // (m2, m3, key), the new second that of (m1, m2, key)
+        const uint32_t key = row_key(d, k, pm);
+        m3 = med3_u32(m2, m3, key);
+        m2 = med3_u32(m1, m2, key);
+        m1 = min(m1, key);
     }
-    return make_float4(m1, m2, m3, __uint_as_float(k1 | (k2 << 16)));
+    return make_uint4(m1, m2, m3, 0u);
 }
 
 // Row geometry of one search for the selection: row length, candidate seq
@@ -583,77 +601,93 @@ constexpr int kStageBatch = 8;  // loads a thread keeps in flight while staging
 // quotient
 __device__ __forceinline__ int div_small(int n, float inv) { return int((float(n) + 0.5f) * inv); }
 
+// whether a distance (by its bits) is at or below the threshold bits and
+// below FLT_MAX (the shader's strict '<' never takes FLT_MAX; NaN never counts)
+__device__ __forceinline__ bool survives(float d, uint32_t thr)
+{
+    const uint32_t b = __float_as_uint(d);
+    return b <= thr && b < kFltMaxBits;
+}
+
 // ---------------------------------------------------------------------------
 // Top-8 (distance, seq) keys of one search by a 16-lane row of the wavefront
-// (sub = lane & 15), from the 3 smallest distances of each of its nrows rows
-// (rt) and its distance rows (db, row stride dcols).  The threshold is the
-// 8th smallest of the stored distances, ties counted: they are distances of
-// the search, so it bounds the search's 8th smallest from above.  Each lane
-// keeps the sorted 8 smallest of its rows (sub, sub+16, ...) and the row
-// takes the 8 smallest heads in turn.  The candidates at or below it are
-// gathered into the row's 16-entry stage: a row whose third smallest is above
-// it holds them among its stored first two, the other rows (usually the
-// best match's row and a neighbour) are scanned.  Returns their number and
-// sets thr; when it is at most 16, lanes 0..7 of the row hold the sorted
+// (sub = lane & 15), from the 3 smallest row keys (row_key, mask pm) of each
+// of its nrows rows (rt) and its distance rows (db, row stride dcols).  The
+// 8th smallest stored key tk, ties counted, belongs to 8 distinct candidates
+// whose distance bits are at most tk | pm, so the threshold thr = tk | pm
+// bounds the search's 8th smallest distance from above.  Each lane keeps the
+// sorted 8 smallest keys of its rows (sub, sub+16, ...) and the row takes
+// the 8 smallest heads in turn.  The candidates with distance bits at most
+// thr are gathered into the row's 16-entry stage: a row whose third key has
+// lower bound above thr holds them among the candidates of its first two
+// keys (read back exactly from the distance row), the other rows (usually
+// the best match's row and a neighbour) are scanned.  Returns their number
+// and sets thr; when it is at most 16, lanes 0..7 of the row hold the sorted
 // top-8 in key/kxy (placeholders where the search has fewer than 8
 // candidates).  nrows is the same for all rows of the wavefront.
 // ---------------------------------------------------------------------------
 template <class Geo>
-__device__ int select16(const float4 *rt, const float *db, int dcols, int nrows, const Geo &geo, bool act, int lane,
-                        unsigned long long *sk, uint32_t *sxy, uint64_t &key, uint32_t &kxy, float &thr)
+__device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, const Geo &geo, bool act, int lane,
+                        unsigned long long *sk, uint32_t *sxy, uint64_t &key, uint32_t &kxy, uint32_t &thr,
+                        uint32_t pm)
 {
     const int grp = lane >> 4, sub = lane & 15;
     const int nslots = (nrows + 15) >> 4;
-    float h[8];
+    uint32_t h[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        h[i] = FLT_MAX;
+        h[i] = ~0u;
     }
     for (int r = sub; r < nrows; r += 16) {
-        const float4 q = rt[r];
-        const float qv[3] = {q.x, q.y, q.z};
+        const uint4 q = rt[r];
+        const uint32_t qv[3] = {q.x, q.y, q.z};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            float v = qv[j];
+            uint32_t v = qv[j];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const float lo = fminf(h[i], v);
-                v = fmaxf(h[i], v);
+                const uint32_t lo = min(h[i], v);
+                v = max(h[i], v);
                 h[i] = lo;
             }
         }
     }
-    thr = FLT_MAX;
+    uint32_t tk = ~0u;
     int taken = 0;
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
-        const float m = row_min16(h[0], lane);
+        const uint32_t m = row_min16_u32(h[0], lane);
         const bool hit = h[0] == m;
         const uint32_t seg = uint32_t(__ballot(hit) >> (grp * 16)) & 0xFFFFu;
         if (taken < 8) {
             taken += __popc(seg);
-            thr = m;
+            tk = m;
         }
         if (hit) {
 #pragma unroll
             for (int i = 0; i < 7; ++i) {
                 h[i] = h[i + 1];
             }
-            h[7] = FLT_MAX;
+            h[7] = ~0u;
         }
     }
-    // stored survivors (2 bits per row slot) and rows to scan (1 bit)
+    thr = tk | pm;
+    // stored survivors (2 bits per row slot) and rows to scan (1 bit); the
+    // row's candidates past its first two keys have distance bits at least
+    // (third key) & ~pm
     uint32_t own = 0, scan = 0;
     int cnt = 0;
     for (int q = 0; q < nslots; ++q) {
         const int r = q * 16 + sub;
         if (act && r < nrows) {
-            const float4 v = rt[r];
-            if (v.z <= thr && v.z < FLT_MAX) {
+            const uint4 v = rt[r];
+            if ((v.z & ~pm) <= thr) {
                 scan |= 1u << q;
             } else {
-                const uint32_t s0 = (v.x <= thr && v.x < FLT_MAX) ? 1u : 0u;
-                const uint32_t s1 = (v.y <= thr && v.y < FLT_MAX) ? 1u : 0u;
+                // here thr < ~pm, so keys passing the bound are real candidates
+                const float *dr = db + r * dcols;
+                const uint32_t s0 = ((v.x & ~pm) <= thr && survives(dr[v.x & pm], thr)) ? 1u : 0u;
+                const uint32_t s1 = ((v.y & ~pm) <= thr && survives(dr[v.y & pm], thr)) ? 1u : 0u;
                 own |= (s0 | (s1 << 1)) << (2 * q);
                 cnt += int(s0 + s1);
             }
@@ -667,17 +701,17 @@ __device__ int select16(const float4 *rt, const float *db, int dcols, int nrows,
             continue;
         }
         const int r = q * 16 + sub;
-        const float4 v = rt[r];
-        const uint32_t kk = __float_as_uint(v.w);
+        const uint4 v = rt[r];
+        const float *dr = db + r * dcols;
         if ((own & 1u) && pos < 16) {
-            const int k = int(kk & 0xFFFFu);
-            sk[pos] = make_key(v.x, geo.seq(r, k));
+            const int k = int(v.x & pm);
+            sk[pos] = make_key(dr[k], geo.seq(r, k));
             sxy[pos] = geo.xy(r, k);
         }
         pos += int(own & 1u);
         if ((own & 2u) && pos < 16) {
-            const int k = int(kk >> 16);
-            sk[pos] = make_key(v.y, geo.seq(r, k));
+            const int k = int(v.y & pm);
+            sk[pos] = make_key(dr[k], geo.seq(r, k));
             sxy[pos] = geo.xy(r, k);
         }
         pos += int((own >> 1) & 1u);
@@ -692,7 +726,7 @@ __device__ int select16(const float4 *rt, const float *db, int dcols, int nrows,
             for (int k0 = 0; k0 < rw; k0 += 16) {
                 const int k = k0 + sub;
                 const float d = k < rw ? db[r * dcols + k] : FLT_MAX;
-                const bool surv = k < rw && d <= thr && d < FLT_MAX;
+                const bool surv = k < rw && survives(d, thr);
                 const uint32_t sm = uint32_t(__ballot(surv) >> (grp * 16)) & 0xFFFFu;
                 const int p = staged + __popc(sm & ((1u << sub) - 1u));
                 if (surv && p < 16) {
@@ -717,20 +751,20 @@ __device__ int select16(const float4 *rt, const float *db, int dcols, int nrows,
     return staged;
 }
 
-// The same search by a whole wavefront, for more than 16 candidates at or
-// below thr (ties): every row holding one is scanned, the candidates staged
-// in batches of 64 and merged into the list.
+// The same search by a whole wavefront, for more than 16 candidates with
+// distance bits at most thr (ties): every row that may hold one (its first
+// key's lower bound at most thr) is scanned, the candidates staged in
+// batches of 64 and merged into the list.
 template <class Geo>
-__device__ void select_wave(const float4 *rt, const float *db, int dcols, int nrows, const Geo &geo, float thr,
-                            int lane, unsigned long long *sk, uint32_t *sxy, TopList &L)
+__device__ void select_wave(const uint4 *rt, const float *db, int dcols, int nrows, const Geo &geo, uint32_t thr,
+                            uint32_t pm, int lane, unsigned long long *sk, uint32_t *sxy, TopList &L)
 {
     list_reset(L);
     int staged = 0;
     for (int r0 = 0; r0 < nrows; r0 += 64) {
         bool any = false;
         if (r0 + lane < nrows) {
-            const float m = rt[r0 + lane].x;
-            any = m <= thr && m < FLT_MAX;
+            any = (rt[r0 + lane].x & ~pm) <= thr;
         }
         uint64_t rows = __ballot(any);
         while (rows) {
@@ -740,7 +774,7 @@ __device__ void select_wave(const float4 *rt, const float *db, int dcols, int nr
             for (int k0 = 0; k0 < rw; k0 += 64) {
                 const int k = k0 + lane;
                 const float d = k < rw ? db[r * dcols + k] : FLT_MAX;
-                const bool surv = k < rw && d <= thr && d < FLT_MAX;
+                const bool surv = k < rw && survives(d, thr);
                 const uint64_t m = __ballot(surv);
                 const int cnt = __popcll(m);
                 if (staged + cnt > 64) {
@@ -912,14 +946,14 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
     const int RH = A.region_rows - 7;
     unsigned long long *stage_key = lds64;                                  // [4][64] survivors
     uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);         // [4][64]
-    // per window row: its 3 smallest distances (x <= y <= z) and the
-    // positions of the first two (w = k1 | k2 << 16)
-    float4 *rowtop = reinterpret_cast<float4 *>(stage_xy + 256);            // [NB][rh]
+    // per window row: its 3 smallest row keys (x <= y <= z)
+    uint4 *rowtop = reinterpret_cast<uint4 *>(stage_xy + 256);              // [NB][rh]
     float *region = reinterpret_cast<float *>(rowtop + NB * RH);            // [region_cols][region_rows]
     float *cur = region + A.region_cols * CS;                               // [NB][64]
     float *dist = cur + NB * 64;                                            // [NB][rh][dist_cols]
     int *stage_over = reinterpret_cast<int *>(dist + NB * RH * A.dist_cols);  // [NB] more than 16 survivors
-    float *stage_thr = reinterpret_cast<float *>(stage_over + NB);             // [NB] their threshold
+    uint32_t *stage_thr = reinterpret_cast<uint32_t *>(stage_over + NB);       // [NB] their threshold
+    const uint32_t pm = pos_mask(A.dist_cols);
 
     const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
     const int run = xcd_block_id();
@@ -977,7 +1011,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int xb = min((bx0 + b) * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         rowtop[b * rh + r] = scan_row_sliding<CM>(cur + b * 64, region + (left - rx0) * CS + r, CS,
-                                                  right - left + 1, dist + (b * rh + r) * A.dist_cols);
+                                                  right - left + 1, dist + (b * rh + r) * A.dist_cols, pm);
     }
     __syncthreads();
 
@@ -1001,9 +1035,10 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const SpatialRows geo{right - left + 1, left, top};
         uint64_t key;
         uint32_t kxy;
-        float thr;
+        uint32_t thr;
         const int staged = select16(rowtop + bs * rh, dist + bs * rh * A.dist_cols, A.dist_cols, rh, geo, act, lane,
-                                    stage_key + wave * 64 + grp * 16, stage_xy + wave * 64 + grp * 16, key, kxy, thr);
+                                    stage_key + wave * 64 + grp * 16, stage_xy + wave * 64 + grp * 16, key, kxy, thr,
+                                    pm);
         if (act && staged <= 16) {
             if (A.spatial) {
                 if (sub < 8) {
@@ -1031,7 +1066,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         const SpatialRows geo{right - left + 1, left, top};
         TopList L;
-        select_wave(rowtop + b * rh, dist + b * rh * A.dist_cols, A.dist_cols, rh, geo, stage_thr[b], lane,
+        select_wave(rowtop + b * rh, dist + b * rh * A.dist_cols, A.dist_cols, rh, geo, stage_thr[b], pm, lane,
                     stage_key + wave * 64, stage_xy + wave * 64, L);
         if (A.spatial) {
             if (lane < 8) {
