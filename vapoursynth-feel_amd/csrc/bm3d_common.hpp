@@ -53,12 +53,17 @@ __host__ __device__ inline long long temporal_lds_bytes(int t_rows, int ps_range
 // positions of the first two (16 bytes), the source region, the blocks, their
 // distance rows (region_rows - 7 window rows at most) and per block an
 // overflow flag and its threshold.
+// column stride of the spatial kernel's column-major source region: odd, so
+// that the staging stores (one column per lane) fall in distinct LDS banks
+__host__ __device__ inline int region_col_stride(int region_rows) { return region_rows | 1; }
+
 __host__ __device__ inline long long rows_lds_bytes(int region_cols, int region_rows, int run_blocks, int dist_cols)
 {
+    // staged positions, per-block flags and thresholds, 3 row keys per
+    // window row, the source region, 16-bit distance rows
     const long long rh = region_rows - 7;
-    return 256LL * 8 + 256LL * 4 + 16LL * run_blocks * rh +
-           4LL * ((long long)region_cols * region_rows + run_blocks * 64LL + run_blocks * rh * dist_cols +
-                  2LL * run_blocks);
+    return 256LL * 4 + 8LL * run_blocks + 12LL * run_blocks * rh +
+           4LL * region_cols * region_col_stride(region_rows) + 2LL * run_blocks * rh * dist_cols;
 }
 
 // Collaborative filtering + weighted scatter of every group.

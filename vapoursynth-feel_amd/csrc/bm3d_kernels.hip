@@ -511,19 +511,20 @@ constexpr uint32_t kFltMaxBits = 0x7F7FFFFFu;  // distances at or above FLT_MAX 
 // ---------------------------------------------------------------------------
 // One window row of the sliding metric from a column-major LDS region (column
 // j of the row starts at col0 + j * CS): column SSDs v_j against reference
-// column min(j, 7) of cb, then
-// D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
-// written to drow[0 .. rw-1].  Returns the row's 3 smallest keys (row_key
-// with mask pm; x <= y <= z, ~0 where the row has fewer candidates).
+// column min(j, 7) (column c of the reference block at cb + c * CS), then
+// D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)).
+// drow[k] gets the upper 16 bits of D(k) (a lower bound of its bits: the
+// selection recomputes the few candidates it keeps).  Returns the row's 3
+// smallest keys (row_key with mask pm; ~0 where the row has fewer).
 // ---------------------------------------------------------------------------
 template <int CM>
-__device__ __forceinline__ uint4 scan_row_sliding(const float *cb, const float *col0, int CS, int rw, float *drow,
+__device__ __forceinline__ uint3 scan_row_sliding(const float *cb, const float *col0, int CS, int rw, uint16_t *drow,
                                                   uint32_t pm)
 {
     float c7[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        c7[i] = cb[56 + i];
+        c7[i] = cb[7 * CS + i];
     }
     uint32_t m1 = ~0u, m2 = ~0u, m3 = ~0u;
     // Column by column with the partial sums of the tree kept rolling:
@@ -540,7 +541,7 @@ __device__ __forceinline__ uint4 scan_row_sliding(const float *cb, const float *
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             sv[i] = sp[i];
-            cc[i] = cb[j * 8 + i];
+            cc[i] = cb[j * CS + i];
         }
         const float v = col_ssd<CM>(cc, sv);
         const float p1 = v_prev + v;    // p1_j-1 (j >= 1)
@@ -573,7 +574,7 @@ __device__ __forceinline__ uint4 scan_row_sliding(const float *cb, const float *
         p2_5 = p2_4;
         p2_4 = p2;
         const uint32_t k = uint32_t(j - 7);
-        drow[k] = d;
+        drow[k] = uint16_t(__float_as_uint(d) >> 16);
         // insertion into m1 <= m2 <= m3: the new third is the median of
         // (m2, m3, key), the new second that of (m1, m2, key)
         const uint32_t key = row_key(d, k, pm);
@@ -581,19 +582,51 @@ __device__ __forceinline__ uint4 scan_row_sliding(const float *cb, const float *
         m2 = med3_u32(m1, m2, key);
         m1 = min(m1, key);
     }
-    return make_uint4(m1, m2, m3, 0u);
+    return make_uint3(m1, m2, m3);
 }
 
-// Row geometry of one search for the selection: row length, candidate seq
-// and position.  A spatial window: rows top.., one length, seq = row*rw + k.
+// D(k) of one window row exactly as scan_row_sliding computes it (same
+// column SSDs, same summation tree)
+template <int CM>
+__device__ __forceinline__ float dist_at(const float *cb, const float *col0, int CS, int k)
+{
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int j = k + i;
+        const float *sp = col0 + j * CS;
+        const float *cp = cb + min(j, 7) * CS;
+        float sv[8], cc[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            sv[e] = sp[e];
+            cc[e] = cp[e];
+        }
+        v[i] = col_ssd<CM>(cc, sv);
+    }
+    return ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
+// lower bound of a distance's bits from its stored upper half
+__device__ __forceinline__ uint32_t lower_bits(uint16_t h) { return uint32_t(h) << 16; }
+
+// Row geometry of one spatial window for the selection: rows top.., one
+// length, seq = row*rw + k, position, and the exact distance of a candidate
+// (reference block at cb, window row 0 starting at col0, column stride CS).
+template <int CM>
 struct SpatialRows {
     int rw, left, top;
+    const float *cb, *col0;
+    int CS;
     __device__ int len(int) const { return rw; }
     __device__ int seq(int row, int k) const { return row * rw + k; }
     __device__ uint32_t xy(int row, int k) const { return uint32_t(left + k) | (uint32_t(top + row) << 16); }
+    __device__ int row_of(uint32_t xy) const { return int(xy >> 16) - top; }
+    __device__ int k_of(uint32_t xy) const { return int(xy & 0xFFFFu) - left; }
+    __device__ float exact(int row, int k) const { return dist_at<CM>(cb, col0 + row, CS, k); }
 };
 
-constexpr int kStageBatch = 8;  // loads a thread keeps in flight while staging
+constexpr int kStageBatch = 16;  // loads a thread keeps in flight while staging
 
 // floor(n / d) for 0 <= n < 2^16 and d >= 1, given inv = 1.0f / d: (n + 1/2)
 // / d lies at least 1/(2d) from an integer, while the float product is off
@@ -612,24 +645,25 @@ __device__ __forceinline__ bool survives(float d, uint32_t thr)
 // ---------------------------------------------------------------------------
 // Top-8 (distance, seq) keys of one search by a 16-lane row of the wavefront
 // (sub = lane & 15), from the 3 smallest row keys (row_key, mask pm) of each
-// of its nrows rows (rt) and its distance rows (db, row stride dcols).  The
-// 8th smallest stored key tk, ties counted, belongs to 8 distinct candidates
-// whose distance bits are at most tk | pm, so the threshold thr = tk | pm
-// bounds the search's 8th smallest distance from above.  Each lane keeps the
-// sorted 8 smallest keys of its rows (sub, sub+16, ...) and the row takes
-// the 8 smallest heads in turn.  The candidates with distance bits at most
-// thr are gathered into the row's 16-entry stage: a row whose third key has
-// lower bound above thr holds them among the candidates of its first two
-// keys (read back exactly from the distance row), the other rows (usually
-// the best match's row and a neighbour) are scanned.  Returns their number
-// and sets thr; when it is at most 16, lanes 0..7 of the row hold the sorted
-// top-8 in key/kxy (placeholders where the search has fewer than 8
-// candidates).  nrows is the same for all rows of the wavefront.
+// of its nrows rows (rt, 3 words per row) and its distance rows (db: upper
+// halves of the distances, row stride dcols).  The 8th smallest stored key
+// tk, ties counted, belongs to 8 distinct candidates whose distance bits are
+// at most tk | pm, so the threshold thr = tk | pm bounds the search's 8th
+// smallest distance from above.  Each lane keeps the sorted 8 smallest keys
+// of its rows (sub, sub+16, ...) and the row takes the 8 smallest heads in
+// turn.  The candidates whose distance bits may be at most thr are gathered
+// into the row's 16-entry stage (their positions): a row whose third key has
+// lower bound above thr holds them among its first two keys, the other rows
+// (usually the best match's row and a neighbour) are scanned by the lower
+// bounds of their distances.  Each lane then recomputes its staged
+// candidate's distance exactly and keeps it if it is at most thr.  Returns
+// the number staged and sets thr; when it is at most 16, lanes 0..7 of the
+// row hold the sorted top-8 in key/kxy (placeholders where the search has
+// fewer than 8 candidates).  nrows is the same for all rows of the wavefront.
 // ---------------------------------------------------------------------------
 template <class Geo>
-__device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, const Geo &geo, bool act, int lane,
-                        unsigned long long *sk, uint32_t *sxy, uint64_t &key, uint32_t &kxy, uint32_t &thr,
-                        uint32_t pm)
+__device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int nrows, const Geo &geo, bool act,
+                        int lane, uint32_t *sxy, uint64_t &key, uint32_t &kxy, uint32_t &thr, uint32_t pm)
 {
     const int grp = lane >> 4, sub = lane & 15;
     const int nslots = (nrows + 15) >> 4;
@@ -639,8 +673,7 @@ __device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, 
         h[i] = ~0u;
     }
     for (int r = sub; r < nrows; r += 16) {
-        const uint4 q = rt[r];
-        const uint32_t qv[3] = {q.x, q.y, q.z};
+        const uint32_t qv[3] = {rt[3 * r], rt[3 * r + 1], rt[3 * r + 2]};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             uint32_t v = qv[j];
@@ -672,22 +705,20 @@ __device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, 
         }
     }
     thr = tk | pm;
-    // stored survivors (2 bits per row slot) and rows to scan (1 bit); the
+    // stored candidates (2 bits per row slot) and rows to scan (1 bit); the
     // row's candidates past its first two keys have distance bits at least
-    // (third key) & ~pm
+    // (third key) & ~pm.  Where a row is not scanned thr < ~pm, so keys
+    // passing the bound are real candidates.
     uint32_t own = 0, scan = 0;
     int cnt = 0;
     for (int q = 0; q < nslots; ++q) {
         const int r = q * 16 + sub;
         if (act && r < nrows) {
-            const uint4 v = rt[r];
-            if ((v.z & ~pm) <= thr) {
+            if ((rt[3 * r + 2] & ~pm) <= thr) {
                 scan |= 1u << q;
             } else {
-                // here thr < ~pm, so keys passing the bound are real candidates
-                const float *dr = db + r * dcols;
-                const uint32_t s0 = ((v.x & ~pm) <= thr && survives(dr[v.x & pm], thr)) ? 1u : 0u;
-                const uint32_t s1 = ((v.y & ~pm) <= thr && survives(dr[v.y & pm], thr)) ? 1u : 0u;
+                const uint32_t s0 = (rt[3 * r] & ~pm) <= thr ? 1u : 0u;
+                const uint32_t s1 = (rt[3 * r + 1] & ~pm) <= thr ? 1u : 0u;
                 own |= (s0 | (s1 << 1)) << (2 * q);
                 cnt += int(s0 + s1);
             }
@@ -701,18 +732,12 @@ __device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, 
             continue;
         }
         const int r = q * 16 + sub;
-        const uint4 v = rt[r];
-        const float *dr = db + r * dcols;
         if ((own & 1u) && pos < 16) {
-            const int k = int(v.x & pm);
-            sk[pos] = make_key(dr[k], geo.seq(r, k));
-            sxy[pos] = geo.xy(r, k);
+            sxy[pos] = geo.xy(r, int(rt[3 * r] & pm));
         }
         pos += int(own & 1u);
         if ((own & 2u) && pos < 16) {
-            const int k = int(v.y & pm);
-            sk[pos] = make_key(dr[k], geo.seq(r, k));
-            sxy[pos] = geo.xy(r, k);
+            sxy[pos] = geo.xy(r, int(rt[3 * r + 1] & pm));
         }
         pos += int((own >> 1) & 1u);
     }
@@ -725,12 +750,10 @@ __device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, 
             const int rw = geo.len(r);
             for (int k0 = 0; k0 < rw; k0 += 16) {
                 const int k = k0 + sub;
-                const float d = k < rw ? db[r * dcols + k] : FLT_MAX;
-                const bool surv = k < rw && survives(d, thr);
-                const uint32_t sm = uint32_t(__ballot(surv) >> (grp * 16)) & 0xFFFFu;
+                const bool maybe = k < rw && lower_bits(db[r * dcols + min(k, rw - 1)]) <= thr;
+                const uint32_t sm = uint32_t(__ballot(maybe) >> (grp * 16)) & 0xFFFFu;
                 const int p = staged + __popc(sm & ((1u << sub) - 1u));
-                if (surv && p < 16) {
-                    sk[p] = make_key(d, geo.seq(r, k));
+                if (maybe && p < 16) {
                     sxy[p] = geo.xy(r, k);
                 }
                 staged += __popc(sm);
@@ -738,10 +761,17 @@ __device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, 
         }
     }
     wave_lds_sync();
-    key = sub < staged && sub < 16 ? sk[sub] : kNoKey;
-    kxy = sub < staged && sub < 16 ? sxy[sub] : 0u;
+    kxy = sub < staged ? sxy[sub] : 0u;
     wave_lds_sync();
+    key = kNoKey;
     if (staged <= 16) {
+        if (sub < staged) {
+            const int r = geo.row_of(kxy), k = geo.k_of(kxy);
+            const float d = geo.exact(r, k);
+            if (survives(d, thr)) {
+                key = make_key(d, geo.seq(r, k));
+            }
+        }
         sort16_dpp(key, kxy, lane);
         if (!(key < kPlaceholderKey)) {
             key = kPlaceholderKey;  // fewer than 8 candidates in the search
@@ -751,20 +781,40 @@ __device__ int select16(const uint4 *rt, const float *db, int dcols, int nrows, 
     return staged;
 }
 
-// The same search by a whole wavefront, for more than 16 candidates with
-// distance bits at most thr (ties): every row that may hold one (its first
-// key's lower bound at most thr) is scanned, the candidates staged in
-// batches of 64 and merged into the list.
+// recomputes the staged candidates (positions in sxy, lanes < staged) and
+// merges the ones at or below thr into the list
 template <class Geo>
-__device__ void select_wave(const uint4 *rt, const float *db, int dcols, int nrows, const Geo &geo, uint32_t thr,
-                            uint32_t pm, int lane, unsigned long long *sk, uint32_t *sxy, TopList &L)
+__device__ __forceinline__ void merge_staged(const Geo &geo, uint32_t thr, int staged, int lane, const uint32_t *sxy,
+                                             TopList &L)
+{
+    wave_lds_sync();
+    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+    wave_lds_sync();
+    uint64_t key = kNoKey;
+    if (lane < staged) {
+        const int r = geo.row_of(kxy), k = geo.k_of(kxy);
+        const float d = geo.exact(r, k);
+        if (survives(d, thr)) {
+            key = make_key(d, geo.seq(r, k));
+        }
+    }
+    list_merge_batch(L, key, kxy, lane);
+}
+
+// The same search by a whole wavefront, for more than 16 staged candidates
+// (ties): every row that may hold one (its first key's lower bound at most
+// thr) is scanned, the candidates staged in batches of 64, recomputed and
+// merged into the list.
+template <class Geo>
+__device__ void select_wave(const uint32_t *rt, const uint16_t *db, int dcols, int nrows, const Geo &geo,
+                            uint32_t thr, uint32_t pm, int lane, uint32_t *sxy, TopList &L)
 {
     list_reset(L);
     int staged = 0;
     for (int r0 = 0; r0 < nrows; r0 += 64) {
         bool any = false;
         if (r0 + lane < nrows) {
-            any = (rt[r0 + lane].x & ~pm) <= thr;
+            any = (rt[3 * (r0 + lane)] & ~pm) <= thr;
         }
         uint64_t rows = __ballot(any);
         while (rows) {
@@ -773,33 +823,23 @@ __device__ void select_wave(const uint4 *rt, const float *db, int dcols, int nro
             const int rw = geo.len(r);
             for (int k0 = 0; k0 < rw; k0 += 64) {
                 const int k = k0 + lane;
-                const float d = k < rw ? db[r * dcols + k] : FLT_MAX;
-                const bool surv = k < rw && survives(d, thr);
-                const uint64_t m = __ballot(surv);
+                const bool maybe = k < rw && lower_bits(db[r * dcols + min(k, rw - 1)]) <= thr;
+                const uint64_t m = __ballot(maybe);
                 const int cnt = __popcll(m);
                 if (staged + cnt > 64) {
-                    wave_lds_sync();
-                    const uint64_t key = lane < staged ? sk[lane] : kNoKey;
-                    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
-                    wave_lds_sync();
-                    list_merge_batch(L, key, kxy, lane);
+                    merge_staged(geo, thr, staged, lane, sxy, L);
                     staged = 0;
                 }
-                if (surv) {
+                if (maybe) {
                     const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
                                                                       __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
-                    sk[p] = make_key(d, geo.seq(r, k));
                     sxy[p] = geo.xy(r, k);
                 }
                 staged += cnt;
             }
         }
     }
-    wave_lds_sync();
-    const uint64_t key = lane < staged ? sk[lane] : kNoKey;
-    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
-    wave_lds_sync();
-    list_merge_batch(L, key, kxy, lane);
+    merge_staged(geo, thr, staged, lane, sxy, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -942,17 +982,15 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 {
     extern __shared__ __align__(16) unsigned long long lds64[];
     const int NB = A.run_blocks;
-    const int CS = A.region_rows;  // column stride of the region
+    const int CS = region_col_stride(A.region_rows);  // column stride of the region
     const int RH = A.region_rows - 7;
-    unsigned long long *stage_key = lds64;                                  // [4][64] survivors
-    uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64 + 256);         // [4][64]
+    uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64);               // [4][64] staged positions
+    int *stage_over = reinterpret_cast<int *>(stage_xy + 256);              // [NB] more than 16 staged
+    uint32_t *stage_thr = reinterpret_cast<uint32_t *>(stage_over + NB);    // [NB] their threshold
     // per window row: its 3 smallest row keys (x <= y <= z)
-    uint4 *rowtop = reinterpret_cast<uint4 *>(stage_xy + 256);              // [NB][rh]
-    float *region = reinterpret_cast<float *>(rowtop + NB * RH);            // [region_cols][region_rows]
-    float *cur = region + A.region_cols * CS;                               // [NB][64]
-    float *dist = cur + NB * 64;                                            // [NB][rh][dist_cols]
-    int *stage_over = reinterpret_cast<int *>(dist + NB * RH * A.dist_cols);  // [NB] more than 16 survivors
-    uint32_t *stage_thr = reinterpret_cast<uint32_t *>(stage_over + NB);       // [NB] their threshold
+    uint32_t *rowtop = stage_thr + NB;                                      // [NB][rh][3]
+    float *region = reinterpret_cast<float *>(rowtop + 3 * NB * RH);        // [region_cols][CS]
+    uint16_t *dist = reinterpret_cast<uint16_t *>(region + A.region_cols * CS);  // [NB][rh][dist_cols]
     const uint32_t pm = pos_mask(A.dist_cols);
 
     const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
@@ -997,11 +1035,6 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
             }
         }
     }
-    for (int i = threadIdx.x; i < nb * 64; i += 256) {
-        const int b = i >> 6, e = i & 63;
-        const int xb = min((bx0 + b) * s, W - 8);
-        cur[i] = plane[(long long)(y + (e & 7)) * pitch + xb + (e >> 3)];  // cur[c*8+i]
-    }
     __syncthreads();
 
     // phase 1: one window row per thread
@@ -1010,8 +1043,13 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int b = t / rh, r = t - b * rh;
         const int xb = min((bx0 + b) * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
-        rowtop[b * rh + r] = scan_row_sliding<CM>(cur + b * 64, region + (left - rx0) * CS + r, CS,
-                                                  right - left + 1, dist + (b * rh + r) * A.dist_cols, pm);
+        // the reference block lies in the region: column c at cb + c * CS
+        const float *cb = region + (xb - rx0) * CS + (y - top);
+        const uint3 t3 = scan_row_sliding<CM>(cb, region + (left - rx0) * CS + r, CS, right - left + 1,
+                                              dist + (b * rh + r) * A.dist_cols, pm);
+        rowtop[3 * (b * rh + r)] = t3.x;
+        rowtop[3 * (b * rh + r) + 1] = t3.y;
+        rowtop[3 * (b * rh + r) + 2] = t3.z;
     }
     __syncthreads();
 
@@ -1032,13 +1070,13 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int bx = bx0 + bs;
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
-        const SpatialRows geo{right - left + 1, left, top};
+        const SpatialRows<CM> geo{right - left + 1, left, top, region + (xb - rx0) * CS + (y - top),
+                                  region + (left - rx0) * CS, CS};
         uint64_t key;
         uint32_t kxy;
         uint32_t thr;
-        const int staged = select16(rowtop + bs * rh, dist + bs * rh * A.dist_cols, A.dist_cols, rh, geo, act, lane,
-                                    stage_key + wave * 64 + grp * 16, stage_xy + wave * 64 + grp * 16, key, kxy, thr,
-                                    pm);
+        const int staged = select16(rowtop + 3 * bs * rh, dist + bs * rh * A.dist_cols, A.dist_cols, rh, geo, act,
+                                    lane, stage_xy + wave * 64 + grp * 16, key, kxy, thr, pm);
         if (act && staged <= 16) {
             if (A.spatial) {
                 if (sub < 8) {
@@ -1064,10 +1102,11 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int bx = bx0 + b;
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
-        const SpatialRows geo{right - left + 1, left, top};
+        const SpatialRows<CM> geo{right - left + 1, left, top, region + (xb - rx0) * CS + (y - top),
+                                  region + (left - rx0) * CS, CS};
         TopList L;
-        select_wave(rowtop + b * rh, dist + b * rh * A.dist_cols, A.dist_cols, rh, geo, stage_thr[b], pm, lane,
-                    stage_key + wave * 64, stage_xy + wave * 64, L);
+        select_wave(rowtop + 3 * b * rh, dist + b * rh * A.dist_cols, A.dist_cols, rh, geo, stage_thr[b], pm, lane,
+                    stage_xy + wave * 64, L);
         if (A.spatial) {
             if (lane < 8) {
                 A.spatial[(long long)(by * A.nx + bx) * 8 + lane] = make_uint2(uint32_t(L.key >> 32), L.xy);
