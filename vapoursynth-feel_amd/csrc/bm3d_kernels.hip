@@ -626,7 +626,7 @@ struct SpatialRows {
     __device__ float exact(int row, int k) const { return dist_at<CM>(cb, col0 + row, CS, k); }
 };
 
-constexpr int kStageBatch = 16;  // loads a thread keeps in flight while staging
+constexpr int kStageBatch = 16;  // loads a lane keeps in flight while staging
 
 // floor(n / d) for 0 <= n < 2^16 and d >= 1, given inv = 1.0f / d: (n + 1/2)
 // / d lies at least 1/(2d) from an integer, while the float product is off
@@ -1007,30 +1007,31 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
     const int rx1 = min(x_last + R, W - 8) + 7;
     const float *plane = A.plane[A.radius];
 
-    // stage the source region (rows top .. bottom+7, columns rx0 .. rx1)
+    // stage the source region (rows top .. bottom+7, columns rx0 .. rx1): a
+    // lane per column, a wavefront per row (every fourth row), each lane
+    // issuing its batch of row loads before storing any of them
     const int ncols = rx1 - rx0 + 1, nrows = rh + 7;
     {
-        // every thread issues its batch of loads before storing any of them
-        const int n = ncols * nrows;
-        const float inv = 1.0f / float(ncols);
-        for (int i0 = threadIdx.x; i0 < n; i0 += 256 * kStageBatch) {
-            float v[kStageBatch];
-            int at[kStageBatch];
+        const int lane = lane_id();
+        const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+        for (int c0 = 0; c0 < ncols; c0 += 64) {
+            const int c = c0 + lane;
+            const bool cok = c < ncols;
+            const float *src = plane + (long long)top * pitch + rx0 + (cok ? c : 0);
+            float *dst = region + (cok ? c : 0) * CS;
+            for (int r0 = wave; r0 < nrows; r0 += 4 * kStageBatch) {
+                float v[kStageBatch];
 #pragma unroll
-            for (int u = 0; u < kStageBatch; ++u) {
-                const int i = i0 + u * 256;
-                at[u] = -1;
-                v[u] = 0.0f;
-                if (i < n) {
-                    const int row = div_small(i, inv), col = i - row * ncols;
-                    v[u] = plane[(long long)(top + row) * pitch + rx0 + col];
-                    at[u] = col * CS + row;
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int r = r0 + 4 * u;
+                    v[u] = (cok && r < nrows) ? src[r * pitch] : 0.0f;
                 }
-            }
 #pragma unroll
-            for (int u = 0; u < kStageBatch; ++u) {
-                if (at[u] >= 0) {
-                    region[at[u]] = v[u];
+                for (int u = 0; u < kStageBatch; ++u) {
+                    const int r = r0 + 4 * u;
+                    if (cok && r < nrows) {
+                        dst[r] = v[u];
+                    }
                 }
             }
         }
