@@ -38,6 +38,9 @@ struct MatchArgs {
     // temporal search after bm3d_match_rows: distance rows per wavefront
     // (ps_num * (2 ps_range + 1)) for the threshold selection, 0: list
     int t_rows;
+    // nonzero: the temporal search runs both directions in one wavefront
+    // (bm3d_match_predictive; needs ps_num * (2 ps_range + 1) <= 32)
+    int pred_pair;
 };
 
 // Dynamic LDS of bm3d_match with the temporal threshold selection: per
@@ -46,6 +49,16 @@ struct MatchArgs {
 __host__ __device__ inline long long temporal_lds_bytes(int t_rows, int ps_range)
 {
     return t_rows > 0 ? 4LL * (64 * 12 + 4LL * t_rows * (2LL * ps_range + 1)) : 0;
+}
+
+// LDS bytes of bm3d_match_predictive (one wavefront per workgroup): 2 x 32
+// survivor keys and positions, the reference block, the 2 ps_num predictor
+// windows of a step ((2 ps_range + 8)^2 floats each, column-major) and a
+// distance row per lane
+__host__ __device__ inline long long predictive_lds_bytes(int ps_num, int ps_range)
+{
+    const long long wh = 2LL * ps_range + 1, wc = wh + 7;
+    return 64LL * 8 + 64LL * 4 + 64LL * 4 + 4LL * 2 * ps_num * wc * wc + 4LL * 64 * wh;
 }
 
 // LDS bytes of bm3d_match_rows: survivor stages of its 4 wavefronts (64 keys

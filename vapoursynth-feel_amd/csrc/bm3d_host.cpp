@@ -137,6 +137,7 @@ struct Instance {
     // workgroup run and the LDS layout; run_blocks == 0 selects the
     // per-wavefront search instead
     int run_blocks = 0, runs_x = 0, region_cols = 0, region_rows = 0, dist_cols = 0;
+    int pred_pair = 0;           // temporal search: both directions per wavefront
     int t_rows = 0;  // temporal threshold selection: distance rows per wavefront (0: list)
     float *dst_dev = nullptr;    // one plane
 
@@ -380,6 +381,7 @@ struct Instance {
         m.region_rows = region_rows;
         m.dist_cols = dist_cols;
         m.t_rows = t_rows;
+        m.pred_pair = pred_pair;
         return m;
     }
 
@@ -713,6 +715,10 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
             const int rows = d.ps_num * wh;
             if (rows <= 64 && temporal_lds_bytes(rows, d.ps_range) <= 48 * 1024) {
                 d.t_rows = rows;
+            }
+            // both directions in one wavefront: a window row per lane of a half
+            if (d.radius <= 2 && rows <= 32 && predictive_lds_bytes(d.ps_num, d.ps_range) <= 64 * 1024) {
+                d.pred_pair = 1;
             }
         }
     }

@@ -454,6 +454,21 @@ __device__ __forceinline__ int row_prefix_sum16(int v)
     return v;
 }
 
+// minimum over the 32 lanes of each half of the wavefront
+__device__ __forceinline__ uint32_t half_min32_u32(uint32_t v, int lane)
+{
+    v = row_min16_u32(v, lane);
+    return min(v, uint32_t(__shfl_xor(int(v), 16)));
+}
+
+// inclusive prefix sum over the 32 lanes of each half of the wavefront
+__device__ __forceinline__ int half_prefix_sum32(int v, int lane)
+{
+    v = row_prefix_sum16(v);
+    const int below = __shfl(v, (lane & ~31) | 15);
+    return (lane & 16) ? v + below : v;
+}
+
 template <int K, int J>
 __device__ __forceinline__ void sort16_step(uint64_t &key, uint32_t &xy, int lane)
 {
@@ -508,23 +523,27 @@ __device__ __forceinline__ uint32_t pos_mask(int n)
 
 constexpr uint32_t kFltMaxBits = 0x7F7FFFFFu;  // distances at or above FLT_MAX never count
 
+// distance row entries: the upper 16 bits of a distance (a lower bound of
+// its bits) or the distance itself
+__device__ __forceinline__ void store_dist(uint16_t *p, float d) { *p = uint16_t(__float_as_uint(d) >> 16); }
+__device__ __forceinline__ void store_dist(float *p, float d) { *p = d; }
+
 // ---------------------------------------------------------------------------
 // One window row of the sliding metric from a column-major LDS region (column
 // j of the row starts at col0 + j * CS): column SSDs v_j against reference
-// column min(j, 7) (column c of the reference block at cb + c * CS), then
-// D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)).
-// drow[k] gets the upper 16 bits of D(k) (a lower bound of its bits: the
-// selection recomputes the few candidates it keeps).  Returns the row's 3
-// smallest keys (row_key with mask pm; ~0 where the row has fewer).
+// column min(j, 7) (column c of the reference block at cb + c * cbs), then
+// D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
+// stored to drow[k] (store_dist).  Returns the row's 3 smallest keys
+// (row_key with mask pm; ~0 where the row has fewer).
 // ---------------------------------------------------------------------------
-template <int CM>
-__device__ __forceinline__ uint3 scan_row_sliding(const float *cb, const float *col0, int CS, int rw, uint16_t *drow,
-                                                  uint32_t pm)
+template <int CM, class DT>
+__device__ __forceinline__ uint3 scan_row_sliding(const float *cb, int cbs, const float *col0, int CS, int rw,
+                                                  DT *drow, uint32_t pm)
 {
     float c7[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        c7[i] = cb[7 * CS + i];
+        c7[i] = cb[7 * cbs + i];
     }
     uint32_t m1 = ~0u, m2 = ~0u, m3 = ~0u;
     // Column by column with the partial sums of the tree kept rolling:
@@ -541,7 +560,7 @@ __device__ __forceinline__ uint3 scan_row_sliding(const float *cb, const float *
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             sv[i] = sp[i];
-            cc[i] = cb[j * CS + i];
+            cc[i] = cb[j * cbs + i];
         }
         const float v = col_ssd<CM>(cc, sv);
         const float p1 = v_prev + v;    // p1_j-1 (j >= 1)
@@ -574,7 +593,7 @@ __device__ __forceinline__ uint3 scan_row_sliding(const float *cb, const float *
         p2_5 = p2_4;
         p2_4 = p2;
         const uint32_t k = uint32_t(j - 7);
-        drow[k] = uint16_t(__float_as_uint(d) >> 16);
+        store_dist(drow + k, d);
         // insertion into m1 <= m2 <= m3: the new third is the median of
         // (m2, m3, key), the new second that of (m1, m2, key)
         const uint32_t key = row_key(d, k, pm);
@@ -1046,7 +1065,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         // the reference block lies in the region: column c at cb + c * CS
         const float *cb = region + (xb - rx0) * CS + (y - top);
-        const uint3 t3 = scan_row_sliding<CM>(cb, region + (left - rx0) * CS + r, CS, right - left + 1,
+        const uint3 t3 = scan_row_sliding<CM>(cb, CS, region + (left - rx0) * CS + r, CS, right - left + 1,
                                               dist + (b * rh + r) * A.dist_cols, pm);
         rowtop[3 * (b * rh + r)] = t3.x;
         rowtop[3 * (b * rh + r) + 1] = t3.y;
@@ -1120,6 +1139,279 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
             finish_group(A, G, by * A.nx + bx, xb, y, lane);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Temporal predictive search after bm3d_match_rows (src/bm3d.comp:540-637),
+// one wavefront per ref block and both directions at once: half 0 (lanes
+// 0..31) follows frames r-1, r-2, .., half 1 frames r+1, r+2, ...  Each step
+// stages the 2 ps_num predictor windows in LDS (a lane per column), walks
+// them a window row per lane (lane h*32 + w*wh + rr: row rr of window w) with
+// the row scan of the spatial search, and selects each half's frame top-8
+// like select16: the 8th smallest stored row key bounds the 8th smallest
+// distance, rows whose third key may be below it are scanned, the others
+// give their first two; the (at most 16) survivors are sorted in the half's
+// first DPP row.  seq of window w's row rr, position k is
+// w*wh^2 + rr*rw_w + k.  The frame top-8s are inserted into the group
+// afterwards in the shader's order (direction -1 for t = 1..r, then +1),
+// which gives the group the shader's sequence of insertions.
+// ---------------------------------------------------------------------------
+
+// the frame top-8 of half h when more than 16 candidates survive (ties): its
+// rows holding one are scanned by the whole wavefront, the survivors merged
+// into a list in batches of 64; returns it in lanes 0..7
+__device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint32_t thr_h, uint32_t pm,
+                                       const float *tdist, int wh, int w, int rr, int rw, int left, int top,
+                                       unsigned long long *sk, uint32_t *sxy, int lane)
+{
+    const int win = wh * wh;
+    TopList L;
+    list_reset(L);
+    int staged = 0;
+    uint32_t rows = uint32_t(__ballot((lane >> 5) == h && row_ok && (m.x & ~pm) <= thr_h) >> (h * 32));
+    while (rows) {
+        const int src = h * 32 + __ffs(rows) - 1;
+        rows &= rows - 1;
+        const int srw = __builtin_amdgcn_readlane(rw, src), sl = __builtin_amdgcn_readlane(left, src);
+        const int st = __builtin_amdgcn_readlane(top, src), sr = __builtin_amdgcn_readlane(rr, src);
+        const int sw = __builtin_amdgcn_readlane(w, src);
+        const float d = lane < srw ? tdist[src * wh + lane] : FLT_MAX;  // srw <= wh <= 32
+        const bool sv = lane < srw && survives(d, thr_h);
+        const uint64_t msk = __ballot(sv);
+        const int cnt = __popcll(msk);
+        if (staged + cnt > 64) {
+            wave_lds_sync();
+            const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+            const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+            wave_lds_sync();
+            list_merge_batch(L, key, kxy, lane);
+            staged = 0;
+        }
+        if (sv) {
+            const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(msk >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo(uint32_t(msk), 0));
+            sk[p] = make_key(d, sw * win + sr * srw + lane);
+            sxy[p] = uint32_t(sl + lane) | (uint32_t(st + sr) << 16);
+        }
+        staged += cnt;
+    }
+    wave_lds_sync();
+    const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+    wave_lds_sync();
+    list_merge_batch(L, key, kxy, lane);
+    return L;
+}
+
+constexpr int kPredBatch = 24;  // window rows a lane loads in one batch
+
+template <int CM>
+__global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
+{
+    extern __shared__ __align__(16) unsigned long long plds[];
+    const int lane = lane_id();
+    const int blk = xcd_block_id();
+    if (blk >= A.nx * A.ny) {
+        return;
+    }
+    const int by = blk / A.nx;
+    const int bx = blk - by * A.nx;
+    const int W = A.width, H = A.height, pitch = A.pitch, r = A.radius;
+    const int x = min(bx * A.step, W - 8);
+    const int y = min(by * A.step, H - 8);
+    const int PR = A.ps_range, PN = A.ps_num;
+    const int wh = 2 * PR + 1, win = wh * wh, wc = wh + 7;
+    const uint32_t pm = pos_mask(wh);
+    unsigned long long *sk = plds;                                // [2][32] survivor keys
+    uint32_t *sxy = reinterpret_cast<uint32_t *>(plds + 64);      // [2][32] their positions
+    float *cur = reinterpret_cast<float *>(sxy + 64);             // cur[c*8+i] = pixel (x+c, y+i)
+    float *regions = cur + 64;                                    // [2 PN][wc columns][wc rows]
+    float *tdist = regions + 2 * PN * wc * wc;                    // [64 lanes][wh]
+
+    cur[lane] = A.plane[r][(long long)(y + (lane & 7)) * pitch + x + (lane >> 3)];
+
+    // the spatial top-8 (bm3d_match_rows) starts the group and predicts
+    TopList L;
+    list_reset(L);
+    {
+        const uint2 e = A.spatial[(long long)blk * 8 + (lane & 7)];
+        if (lane < 8) {
+            L.key = uint64_t(e.x) << 32;
+            L.xy = e.y;
+        }
+    }
+    GroupList G;
+    G.e = __uint_as_float(uint32_t(L.key >> 32));
+    G.xy = L.xy;
+    G.z = r;
+
+    const int half = lane >> 5, sl = lane & 31;
+    const bool slot = sl < PN * wh;
+    const int w = slot ? div_small(sl, 1.0f / float(wh)) : 0;
+    const int rr = sl - w * wh;
+    uint32_t pred = uint32_t(__shfl(int(L.xy), sl & 7));  // lanes h*32 + i: predictor i
+    uint64_t fkey[2] = {kNoKey, kNoKey};  // frame top-8 of steps 1, 2 (lanes h*32 + 0..7)
+    uint32_t fxy[2] = {0u, 0u};
+#pragma unroll
+    for (int t = 1; t <= 2; ++t) {
+        if (t > r) {
+            break;
+        }
+        const float *plane = A.plane[half ? r + t : r - t];
+        // stage: lane h*32 + c loads column c of window (h, p), every row
+        for (int p = 0; p < PN; ++p) {
+            const uint32_t q = uint32_t(__shfl(int(pred), half * 32 + p));
+            const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
+            const int left = max(cx - PR, 0), top = max(cy - PR, 0);
+            const int ncol = min(cx + PR, W - 8) - left + 8;
+            const int nrow = min(cy + PR, H - 8) - top + 8;
+            const bool cok = sl < ncol;
+            const float *src = plane + (long long)top * pitch + left + (cok ? sl : 0);
+            float *dst = regions + ((half * PN + p) * wc + (cok ? sl : 0)) * wc;
+            for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
+                float v[kPredBatch];
+#pragma unroll
+                for (int u = 0; u < kPredBatch; ++u) {
+                    const int i = i0 + u;
+                    v[u] = (cok && i < nrow) ? src[(long long)i * pitch] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < kPredBatch; ++u) {
+                    const int i = i0 + u;
+                    if (cok && i < nrow) {
+                        dst[i] = v[u];
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+        // scan: lane h*32 + w*wh + rr walks row rr of window (h, w)
+        const uint32_t q = uint32_t(__shfl(int(pred), half * 32 + w));
+        const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
+        const int left = max(cx - PR, 0), top = max(cy - PR, 0);
+        const int rw = min(cx + PR, W - 8) - left + 1;
+        const int rh = min(cy + PR, H - 8) - top + 1;
+        const bool row_ok = slot && rr < rh;
+        float *drow = tdist + lane * wh;
+        uint3 m = make_uint3(~0u, ~0u, ~0u);
+        if (row_ok) {
+            m = scan_row_sliding<CM>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
+        }
+        wave_lds_sync();
+        // threshold: the 8th smallest stored key of the half, ties counted
+        uint32_t h0 = m.x, h1 = m.y, h2 = m.z, tk = ~0u;
+        int taken = 0;
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const uint32_t mn = half_min32_u32(h0, lane);
+            const bool hit = h0 == mn;
+            const int cnt = __popc(uint32_t(__ballot(hit) >> (half * 32)));
+            if (taken < 8) {
+                taken += cnt;
+                tk = mn;
+            }
+            if (hit) {
+                h0 = h1;
+                h1 = h2;
+                h2 = ~0u;
+            }
+        }
+        const uint32_t thr = tk | pm;
+        // survivors: a row whose third key's lower bound is above thr holds
+        // them among its first two (then thr < ~pm: those are real keys)
+        const bool scan = row_ok && (m.z & ~pm) <= thr;
+        bool s0 = false, s1 = false;
+        float d0 = 0.0f, d1 = 0.0f;
+        if (row_ok && !scan) {
+            if ((m.x & ~pm) <= thr) {
+                d0 = drow[m.x & pm];
+                s0 = survives(d0, thr);
+            }
+            if ((m.y & ~pm) <= thr) {
+                d1 = drow[m.y & pm];
+                s1 = survives(d1, thr);
+            }
+        }
+        unsigned long long *hk = sk + half * 32;
+        uint32_t *hxy = sxy + half * 32;
+        const int cnt = int(s0) + int(s1);
+        const int incl = half_prefix_sum32(cnt, lane);
+        int pos = incl - cnt;
+        int staged = __shfl(incl, half * 32 + 31);
+        const int seq0 = w * win + rr * rw;
+        if (s0 && pos < 32) {
+            hk[pos] = make_key(d0, seq0 + int(m.x & pm));
+            hxy[pos] = uint32_t(left + int(m.x & pm)) | (uint32_t(top + rr) << 16);
+        }
+        pos += int(s0);
+        if (s1 && pos < 32) {
+            hk[pos] = make_key(d1, seq0 + int(m.y & pm));
+            hxy[pos] = uint32_t(left + int(m.y & pm)) | (uint32_t(top + rr) << 16);
+        }
+        // the scanned rows, lowest first in each half
+        uint32_t rows = uint32_t(__ballot(scan) >> (half * 32));
+        while (__ballot(rows != 0u)) {
+            const bool has = rows != 0u;
+            const int src = half * 32 + (has ? __ffs(rows) - 1 : 0);
+            if (has) {
+                rows &= rows - 1u;
+            }
+            const int srw = __shfl(rw, src), sleft = __shfl(left, src), stop = __shfl(top, src);
+            const int srr = __shfl(rr, src), sw = __shfl(w, src);
+            const float d = has && sl < srw ? tdist[src * wh + sl] : FLT_MAX;  // srw <= wh <= 32
+            const bool sv = has && sl < srw && survives(d, thr);
+            const uint32_t sm = uint32_t(__ballot(sv) >> (half * 32));
+            const int p = staged + __popc(sm & ((1u << sl) - 1u));
+            if (sv && p < 32) {
+                hk[p] = make_key(d, sw * win + srr * srw + sl);
+                hxy[p] = uint32_t(sleft + sl) | (uint32_t(stop + srr) << 16);
+            }
+            staged += __popc(sm);
+        }
+        wave_lds_sync();
+        uint64_t key = sl < staged && sl < 16 ? hk[sl] : kNoKey;
+        uint32_t kxy = sl < staged && sl < 16 ? hxy[sl] : 0u;
+        wave_lds_sync();
+        sort16_dpp(key, kxy, lane);
+        if (!(key < kPlaceholderKey)) {
+            key = kPlaceholderKey;  // fewer than 8 candidates in the step
+            kxy = 0u;
+        }
+        // a half with more than 16 survivors: the whole-wavefront merge
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (__builtin_amdgcn_readlane(staged, h * 32) > 16) {
+                const TopList F = predictive_fallback(h, m, row_ok, uint32_t(__builtin_amdgcn_readlane(int(thr), h * 32)),
+                                                      pm, tdist, wh, w, rr, rw, left, top, sk, sxy, lane);
+                const uint64_t fk = (uint64_t(uint32_t(__shfl(int(uint32_t(F.key >> 32)), sl & 7))) << 32) |
+                                    uint32_t(__shfl(int(uint32_t(F.key)), sl & 7));
+                const uint32_t fx = uint32_t(__shfl(int(F.xy), sl & 7));
+                if (half == h && sl < 8) {
+                    key = fk;
+                    kxy = fx;
+                }
+            }
+        }
+        fkey[t - 1] = key;
+        fxy[t - 1] = kxy;
+        pred = kxy;
+    }
+    // the frame top-8s into the group in the shader's order
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int t = 1; t <= 2; ++t) {
+            if (t > r) {
+                break;
+            }
+            for (int i = 0; i < PN; ++i) {
+                const uint64_t fk = readlane_u64(fkey[t - 1], h * 32 + i);
+                const uint32_t fx = readlane_u32(fxy[t - 1], h * 32 + i);
+                group_insert(G, __uint_as_float(uint32_t(fk >> 32)), fx, h ? r + t : r - t, lane);
+            }
+        }
+    }
+    finish_group(A, G, blk, x, y, lane);
 }
 
 template <int CM, bool EXACT>
@@ -1829,6 +2121,15 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
             hipLaunchKernelGGL((bm3d_match_rows<kFpExplicit>), rgrid, block, lds, s, rows);
         }
         if (a.radius == 0) {
+            return hipGetLastError();
+        }
+        if (a.pred_pair) {
+            const size_t plds = size_t(predictive_lds_bytes(a.ps_num, a.ps_range));
+            if (fused) {
+                hipLaunchKernelGGL((bm3d_match_predictive<kFpAll>), dim3(nref), dim3(64), plds, s, a);
+            } else {
+                hipLaunchKernelGGL((bm3d_match_predictive<kFpExplicit>), dim3(nref), dim3(64), plds, s, a);
+            }
             return hipGetLastError();
         }
         const size_t tlds = size_t(temporal_lds_bytes(a.t_rows, a.ps_range));
