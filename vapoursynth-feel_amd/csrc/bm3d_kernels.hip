@@ -1257,29 +1257,33 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         if (t > r) {
             break;
         }
-        const float *plane = A.plane[half ? r + t : r - t];
-        // stage: lane h*32 + c loads column c of window (h, p), every row
-        for (int p = 0; p < PN; ++p) {
-            const uint32_t q = uint32_t(__shfl(int(pred), half * 32 + p));
-            const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
-            const int left = max(cx - PR, 0), top = max(cy - PR, 0);
-            const int ncol = min(cx + PR, W - 8) - left + 8;
-            const int nrow = min(cy + PR, H - 8) - top + 8;
-            const bool cok = sl < ncol;
-            const float *src = plane + (long long)top * pitch + left + (cok ? sl : 0);
-            float *dst = regions + ((half * PN + p) * wc + (cok ? sl : 0)) * wc;
-            for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
-                float v[kPredBatch];
+        // stage the windows of one direction at a time (one frame): lane
+        // 32*j + c loads column c of window (h, p0 + j), every row; rows and
+        // columns past the window repeat its last one (never read)
+        for (int h = 0; h < 2; ++h) {
+            const float *plane = A.plane[h ? r + t : r - t];
+            for (int p0 = 0; p0 < PN; p0 += 2) {
+                const int p = p0 + half;
+                const uint32_t q = uint32_t(__shfl(int(pred), h * 32 + min(p, PN - 1)));
+                const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
+                const int left = max(cx - PR, 0), top = max(cy - PR, 0);
+                const int ncol = min(cx + PR, W - 8) - left + 8;
+                const int nrow = min(cy + PR, H - 8) - top + 8;
+                const uint32_t base = uint32_t(top * pitch + left + min(sl, ncol - 1));
+                float *dst = regions + ((h * PN + p) * wc + sl) * wc;
+                if (p < PN && sl < wc) {
+                    for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
+                        float v[kPredBatch];
 #pragma unroll
-                for (int u = 0; u < kPredBatch; ++u) {
-                    const int i = i0 + u;
-                    v[u] = (cok && i < nrow) ? src[(long long)i * pitch] : 0.0f;
-                }
+                        for (int u = 0; u < kPredBatch; ++u) {
+                            v[u] = plane[base + uint32_t(min(i0 + u, nrow - 1) * pitch)];
+                        }
 #pragma unroll
-                for (int u = 0; u < kPredBatch; ++u) {
-                    const int i = i0 + u;
-                    if (cok && i < nrow) {
-                        dst[i] = v[u];
+                        for (int u = 0; u < kPredBatch; ++u) {
+                            if (i0 + u < wc) {
+                                dst[i0 + u] = v[u];
+                            }
+                        }
                     }
                 }
             }
