@@ -454,18 +454,23 @@ __device__ __forceinline__ int row_prefix_sum16(int v)
     return v;
 }
 
-// minimum over the 32 lanes of each half of the wavefront
+// minimum over the 32 lanes of each half of the wavefront: the row minimum,
+// then v_permlane16_swap, which leaves every lane its own row's value and
+// the other row's of the pair in its two results
 __device__ __forceinline__ uint32_t half_min32_u32(uint32_t v, int lane)
 {
     v = row_min16_u32(v, lane);
-    return min(v, uint32_t(__shfl_xor(int(v), 16)));
+    const auto sw = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return min(uint32_t(sw[0]), uint32_t(sw[1]));
 }
 
-// inclusive prefix sum over the 32 lanes of each half of the wavefront
+// inclusive prefix sum over the 32 lanes of each half of the wavefront: the
+// odd row adds the even row's total (row_bcast:15 hands lane 15 of each row
+// to the next row)
 __device__ __forceinline__ int half_prefix_sum32(int v, int lane)
 {
     v = row_prefix_sum16(v);
-    const int below = __shfl(v, (lane & ~31) | 15);
+    const int below = int(__builtin_amdgcn_update_dpp(0u, uint32_t(v), 0x142, 0xA, 0xF, false));
     return (lane & 16) ? v + below : v;
 }
 
@@ -1257,31 +1262,41 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         if (t > r) {
             break;
         }
-        // stage the windows of one direction at a time (one frame): lane
-        // 32*j + c loads column c of window (h, p0 + j), every row; rows and
-        // columns past the window repeat its last one (never read)
-        for (int h = 0; h < 2; ++h) {
-            const float *plane = A.plane[h ? r + t : r - t];
-            for (int p0 = 0; p0 < PN; p0 += 2) {
-                const int p = p0 + half;
+        // stage the windows: lane 32*j + c loads column c of window
+        // (h, p0 + j) of both directions h, every row, before storing any;
+        // rows and columns past a window repeat its last one (never read)
+        for (int p0 = 0; p0 < PN; p0 += 2) {
+            const int p = p0 + half;
+            const bool on = p < PN && sl < wc;
+            float v[2][kPredBatch];
+            uint32_t base[2];
+            int nrow[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
                 const uint32_t q = uint32_t(__shfl(int(pred), h * 32 + min(p, PN - 1)));
                 const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
                 const int left = max(cx - PR, 0), top = max(cy - PR, 0);
                 const int ncol = min(cx + PR, W - 8) - left + 8;
-                const int nrow = min(cy + PR, H - 8) - top + 8;
-                const uint32_t base = uint32_t(top * pitch + left + min(sl, ncol - 1));
-                float *dst = regions + ((h * PN + p) * wc + sl) * wc;
-                if (p < PN && sl < wc) {
-                    for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
-                        float v[kPredBatch];
+                nrow[h] = min(cy + PR, H - 8) - top + 8;
+                base[h] = uint32_t(top * pitch + left + min(sl, ncol - 1));
+            }
+            for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
+                if (on) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float *plane = A.plane[h ? r + t : r - t];
 #pragma unroll
                         for (int u = 0; u < kPredBatch; ++u) {
-                            v[u] = plane[base + uint32_t(min(i0 + u, nrow - 1) * pitch)];
+                            v[h][u] = plane[base[h] + uint32_t(min(i0 + u, nrow[h] - 1) * pitch)];
                         }
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        float *dst = regions + ((h * PN + p) * wc + sl) * wc;
 #pragma unroll
                         for (int u = 0; u < kPredBatch; ++u) {
                             if (i0 + u < wc) {
-                                dst[i0 + u] = v[u];
+                                dst[i0 + u] = v[h][u];
                             }
                         }
                     }
@@ -1360,8 +1375,11 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
             if (has) {
                 rows &= rows - 1u;
             }
-            const int srw = __shfl(rw, src), sleft = __shfl(left, src), stop = __shfl(top, src);
-            const int srr = __shfl(rr, src), sw = __shfl(w, src);
+            // the row's geometry in two shuffles: left | top << 16, rw | rr << 8 | w << 16
+            const uint32_t g0 = uint32_t(__shfl(int(uint32_t(left) | (uint32_t(top) << 16)), src));
+            const uint32_t g1 = uint32_t(__shfl(int(uint32_t(rw) | (uint32_t(rr) << 8) | (uint32_t(w) << 16)), src));
+            const int sleft = int(g0 & 0xFFFFu), stop = int(g0 >> 16);
+            const int srw = int(g1 & 0xFFu), srr = int((g1 >> 8) & 0xFFu), sw = int(g1 >> 16);
             const float d = has && sl < srw ? tdist[src * wh + sl] : FLT_MAX;  // srw <= wh <= 32
             const bool sv = has && sl < srw && survives(d, thr);
             const uint32_t sm = uint32_t(__ballot(sv) >> (half * 32));
