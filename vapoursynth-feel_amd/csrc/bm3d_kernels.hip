@@ -325,9 +325,10 @@ __device__ __forceinline__ void group_insert(GroupList &G, float fe, uint32_t fx
     }
     const uint64_t before = __ballot(lane < 8 && G.e <= fe);
     const int pos = __popcll(before & 0xFFull);
-    const float up_e = __shfl_up(G.e, 1);
-    const uint32_t up_xy = uint32_t(__shfl_up(int(G.xy), 1));
-    const int up_z = __shfl_up(G.z, 1);
+    // lanes 1..7 take their left neighbour's entry (DPP row_shr:1)
+    const float up_e = __uint_as_float(dpp_row_shr1(__float_as_uint(G.e)));
+    const uint32_t up_xy = dpp_row_shr1(G.xy);
+    const int up_z = int(dpp_row_shr1(uint32_t(G.z)));
     if (lane == pos) {
         G.e = fe;
         G.xy = fxy;
@@ -345,9 +346,9 @@ __device__ __forceinline__ void finish_group(const MatchArgs &A, GroupList &G, i
     const uint32_t self_xy = uint32_t(x) | (uint32_t(y) << 16);
     const uint64_t present = __ballot(lane < 8 && G.xy == self_xy && G.z == A.radius);
     if (present == 0) {
-        const float up_e = __shfl_up(G.e, 1);
-        const uint32_t up_xy = uint32_t(__shfl_up(int(G.xy), 1));
-        const int up_z = __shfl_up(G.z, 1);
+        const float up_e = __uint_as_float(dpp_row_shr1(__float_as_uint(G.e)));
+        const uint32_t up_xy = dpp_row_shr1(G.xy);
+        const int up_z = int(dpp_row_shr1(uint32_t(G.z)));
         if (lane == 0) {
             G.e = -1.0f;
             G.xy = self_xy;
