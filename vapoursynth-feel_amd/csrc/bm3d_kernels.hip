@@ -1153,12 +1153,14 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 // 0..31) follows frames r-1, r-2, .., half 1 frames r+1, r+2, ...  Each step
 // stages the 2 ps_num predictor windows in LDS (a lane per column), walks
 // them a window row per lane (lane h*32 + w*wh + rr: row rr of window w) with
-// the row scan of the spatial search, and selects each half's frame top-8
-// like select16: the 8th smallest stored row key bounds the 8th smallest
-// distance, rows whose third key may be below it are scanned, the others
-// give their first two; the (at most 16) survivors are sorted in the half's
-// first DPP row.  seq of window w's row rr, position k is
-// w*wh^2 + rr*rw_w + k.  The frame top-8s are inserted into the group
+// the row scan of the spatial search, and selects each half's first ps_num
+// (distance, seq) keys -- all the shader uses of a step's frame top-8: the
+// next step's predictors and the group insertions (src/bm3d.comp:557-562,
+// :624-632) -- like select16: the ps_num-th smallest stored row key bounds
+// the ps_num-th smallest distance, rows whose third key may be below it are
+// scanned, the others give their first two; the survivors (at most 16) are
+// sorted in the half's first DPP row.  seq of window w's row rr, position k
+// is w*wh^2 + rr*rw_w + k.  The steps' keys are inserted into the group
 // afterwards in the shader's order (direction -1 for t = 1..r, then +1),
 // which gives the group the shader's sequence of insertions.
 // ---------------------------------------------------------------------------
@@ -1270,7 +1272,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
             const int p = p0 + half;
             const bool on = p < PN && sl < wc;
             float v[2][kPredBatch];
-            uint32_t base[2];
+            uint32_t base[2];  // byte offsets in the frame (a plane is below 4 GiB)
             int nrow[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -1279,16 +1281,18 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
                 const int left = max(cx - PR, 0), top = max(cy - PR, 0);
                 const int ncol = min(cx + PR, W - 8) - left + 8;
                 nrow[h] = min(cy + PR, H - 8) - top + 8;
-                base[h] = uint32_t(top * pitch + left + min(sl, ncol - 1));
+                base[h] = uint32_t(top * pitch + left + min(sl, ncol - 1)) * 4u;
             }
+            const uint32_t pitch4 = uint32_t(pitch) * 4u;
             for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
                 if (on) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        const float *plane = A.plane[h ? r + t : r - t];
+                        const char *plane = reinterpret_cast<const char *>(A.plane[h ? r + t : r - t]);
 #pragma unroll
                         for (int u = 0; u < kPredBatch; ++u) {
-                            v[h][u] = plane[base[h] + uint32_t(min(i0 + u, nrow[h] - 1) * pitch)];
+                            const uint32_t off = base[h] + uint32_t(min(i0 + u, nrow[h] - 1)) * pitch4;
+                            v[h][u] = *reinterpret_cast<const float *>(plane + off);
                         }
                     }
 #pragma unroll
@@ -1318,15 +1322,16 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
             m = scan_row_sliding<CM>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
         }
         wave_lds_sync();
-        // threshold: the 8th smallest stored key of the half, ties counted
+        // only the step's first ps_num (distance, seq) keys are used (the
+        // next predictors and the group insertions): the threshold is the
+        // ps_num-th smallest stored key of the half, ties counted
         uint32_t h0 = m.x, h1 = m.y, h2 = m.z, tk = ~0u;
         int taken = 0;
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
+        for (int it = 0; it < PN; ++it) {
             const uint32_t mn = half_min32_u32(h0, lane);
             const bool hit = h0 == mn;
             const int cnt = __popc(uint32_t(__ballot(hit) >> (half * 32)));
-            if (taken < 8) {
+            if (taken < PN) {
                 taken += cnt;
                 tk = mn;
             }
@@ -1395,7 +1400,14 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         uint64_t key = sl < staged && sl < 16 ? hk[sl] : kNoKey;
         uint32_t kxy = sl < staged && sl < 16 ? hxy[sl] : 0u;
         wave_lds_sync();
-        sort16_dpp(key, kxy, lane);
+        if (__builtin_amdgcn_readlane(staged, 0) <= 4 && __builtin_amdgcn_readlane(staged, 32) <= 4) {
+            // the usual case: a 4-lane bitonic sort
+            sort16_step<2, 1>(key, kxy, lane);
+            sort16_step<4, 2>(key, kxy, lane);
+            sort16_step<4, 1>(key, kxy, lane);
+        } else {
+            sort16_dpp(key, kxy, lane);
+        }
         if (!(key < kPlaceholderKey)) {
             key = kPlaceholderKey;  // fewer than 8 candidates in the step
             kxy = 0u;
