@@ -542,7 +542,7 @@ __device__ __forceinline__ void store_dist(float *p, float d) { *p = d; }
 // stored to drow[k] (store_dist).  Returns the row's 3 smallest keys
 // (row_key with mask pm; ~0 where the row has fewer).
 // ---------------------------------------------------------------------------
-template <int CM, class DT>
+template <int CM, int UNROLL, class DT>
 __device__ __forceinline__ uint3 scan_row_sliding(const float *cb, int cbs, const float *col0, int CS, int rw,
                                                   DT *drow, uint32_t pm)
 {
@@ -579,7 +579,7 @@ __device__ __forceinline__ uint3 scan_row_sliding(const float *cb, int cbs, cons
         p2_5 = p2_4;
         p2_4 = p2;
     }
-#pragma unroll 4
+#pragma unroll UNROLL
     for (int j = 7; j <= rw + 6; ++j) {
         const float *sp = col0 + j * CS;
         float sv[8];
@@ -1071,7 +1071,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         // the reference block lies in the region: column c at cb + c * CS
         const float *cb = region + (xb - rx0) * CS + (y - top);
-        const uint3 t3 = scan_row_sliding<CM>(cb, CS, region + (left - rx0) * CS + r, CS, right - left + 1,
+        const uint3 t3 = scan_row_sliding<CM, 8>(cb, CS, region + (left - rx0) * CS + r, CS, right - left + 1,
                                               dist + (b * rh + r) * A.dist_cols, pm);
         rowtop[3 * (b * rh + r)] = t3.x;
         rowtop[3 * (b * rh + r) + 1] = t3.y;
@@ -1319,7 +1319,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         float *drow = tdist + lane * wh;
         uint3 m = make_uint3(~0u, ~0u, ~0u);
         if (row_ok) {
-            m = scan_row_sliding<CM>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
+            m = scan_row_sliding<CM, 4>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
         }
         wave_lds_sync();
         // only the step's first ps_num (distance, seq) keys are used (the
