@@ -419,19 +419,27 @@ __device__ __forceinline__ float wave_min(float v)
 }
 
 // lane value of the partner lane ^ J inside a 16-lane row
+// DPP move with bound_ctrl: a lane without a source lane reads 0 (no old
+// value to carry, so no copy before the move)
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), CTRL, 0xF, 0xF, true));
+}
+
 template <int J>
 __device__ __forceinline__ uint32_t dpp_xor(uint32_t v, int lane)
 {
     if (J == 1) {
-        return dpp_u32<0xB1>(v, v);  // quad_perm [1,0,3,2]
+        return dpp_mov<0xB1>(v);  // quad_perm [1,0,3,2]
     } else if (J == 2) {
-        return dpp_u32<0x4E>(v, v);  // quad_perm [2,3,0,1]
+        return dpp_mov<0x4E>(v);  // quad_perm [2,3,0,1]
     } else if (J == 4) {
-        const uint32_t up = dpp_u32<0x104>(v, v);    // row_shl:4 (lane i reads i+4)
-        const uint32_t down = dpp_u32<0x114>(v, v);  // row_shr:4 (lane i reads i-4)
+        const uint32_t up = dpp_mov<0x104>(v);    // row_shl:4 (lane i reads i+4)
+        const uint32_t down = dpp_mov<0x114>(v);  // row_shr:4 (lane i reads i-4)
         return (lane & 4) ? down : up;
     } else {
-        return dpp_u32<0x128>(v, v);  // row_ror:8 (lane i reads i^8)
+        return dpp_mov<0x128>(v);  // row_ror:8 (lane i reads i^8)
     }
 }
 
@@ -448,10 +456,10 @@ __device__ __forceinline__ uint32_t row_min16_u32(uint32_t v, int lane)
 // inclusive prefix sum over the 16 lanes of each row (row_shr with zero fill)
 __device__ __forceinline__ int row_prefix_sum16(int v)
 {
-    v += int(dpp_u32<0x111>(0u, uint32_t(v)));  // row_shr:1
-    v += int(dpp_u32<0x112>(0u, uint32_t(v)));  // row_shr:2
-    v += int(dpp_u32<0x114>(0u, uint32_t(v)));  // row_shr:4
-    v += int(dpp_u32<0x118>(0u, uint32_t(v)));  // row_shr:8
+    v += int(dpp_mov<0x111>(uint32_t(v)));  // row_shr:1
+    v += int(dpp_mov<0x112>(uint32_t(v)));  // row_shr:2
+    v += int(dpp_mov<0x114>(uint32_t(v)));  // row_shr:4
+    v += int(dpp_mov<0x118>(uint32_t(v)));  // row_shr:8
     return v;
 }
 
@@ -484,11 +492,11 @@ __device__ __forceinline__ void sort16_step(uint64_t &key, uint32_t &xy, int lan
     const uint64_t ok = (uint64_t(ohi) << 32) | olo;
     const bool asc = ((lane & 15) & K) == 0;
     const bool lower = (lane & J) == 0;
-    const bool take = (lower == asc) ? (ok < key) : (ok > key);
-    if (take) {
-        key = ok;
-        xy = oxy;
-    }
+    const bool lt = ok < key, gt = ok > key;
+    const bool take = (lt && lower == asc) || (gt && lower != asc);
+    // selects, not a branch: three v_cndmask under one lane mask
+    key = (uint64_t(take ? ohi : uint32_t(key >> 32)) << 32) | (take ? olo : uint32_t(key));
+    xy = take ? oxy : xy;
 }
 
 // bitonic sort, ascending, of the 16 keys of each 16-lane row
