@@ -1439,19 +1439,40 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         fxy[t - 1] = kxy;
         pred = kxy;
     }
-    // the frame top-8s into the group in the shader's order
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-#pragma unroll
-        for (int t = 1; t <= 2; ++t) {
-            if (t > r) {
-                break;
-            }
-            for (int i = 0; i < PN; ++i) {
-                const uint64_t fk = readlane_u64(fkey[t - 1], h * 32 + i);
-                const uint32_t fx = readlane_u32(fxy[t - 1], h * 32 + i);
-                group_insert(G, __uint_as_float(uint32_t(fk >> 32)), fx, h ? r + t : r - t, lane);
-            }
+    // The steps' keys into the group in the shader's order (direction -1
+    // for t = 1..r, then +1; i < ps_num).  Inserting entries one by one with
+    // ginsert8's strict '<' leaves the 8 smallest of the group and the
+    // entries by (distance, rank), where the group's members rank first in
+    // their order and the entries after them in insertion order; so 8
+    // entries at a time are merged by one 16-lane sort of (distance bits,
+    // rank, z) keys: lanes 0..7 the group, lanes 8..15 the entries.
+    const int n_ins = 2 * r * PN;
+    for (int c0 = 0; c0 < n_ins; c0 += 8) {
+        const int e_idx = c0 + (lane - 8);
+        const bool entry = lane >= 8 && lane < 16 && e_idx < n_ins;
+        const int ei = entry ? e_idx : 0;
+        const int ht = ei / PN, i = ei - ht * PN;
+        const int h = ht / r, t = ht - h * r + 1;
+        const int src = h * 32 + i;
+        const uint32_t e1 = uint32_t(__shfl(int(uint32_t(fkey[0] >> 32)), src));
+        const uint32_t e2 = uint32_t(__shfl(int(uint32_t(fkey[1] >> 32)), src));
+        const uint32_t x1 = uint32_t(__shfl(int(fxy[0]), src));
+        const uint32_t x2 = uint32_t(__shfl(int(fxy[1]), src));
+        uint64_t key = kNoKey;
+        uint32_t kxy = 0u;
+        if (lane < 8) {
+            key = (uint64_t(__float_as_uint(G.e)) << 32) | (uint32_t(lane) << 8) | uint32_t(G.z);
+            kxy = G.xy;
+        } else if (entry) {
+            const uint32_t z = uint32_t(h ? r + t : r - t);
+            key = (uint64_t(t == 1 ? e1 : e2) << 32) | (uint32_t(lane) << 8) | z;
+            kxy = t == 1 ? x1 : x2;
+        }
+        sort16_dpp(key, kxy, lane);
+        if (lane < 8) {
+            G.e = __uint_as_float(uint32_t(key >> 32));
+            G.xy = kxy;
+            G.z = int(key & 0xFFu);
         }
     }
     finish_group(A, G, blk, x, y, lane);
