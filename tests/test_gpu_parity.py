@@ -63,6 +63,12 @@ CASES = {
     # workgroup of the temporal search
     "wide_predictive": (lambda s: s.shifted_clip(12, 5, 120, 96, 3 / 255, (2.0, -1.0)),
                         dict(sigma=3.0, block_step=6, bm_range=8, radius=2, ps_num=3, ps_range=12)),
+    # one predictor per step (one staging half idle), 4 group entries
+    "one_predictor": (lambda s: s.shifted_clip(14, 4, 80, 64, 3 / 255, (1.5, -0.5)),
+                      dict(sigma=3.0, block_step=4, bm_range=6, radius=2, ps_num=1, ps_range=5)),
+    # four predictors per step (two staging passes), 8 group entries
+    "four_predictors": (lambda s: s.shifted_clip(15, 4, 88, 72, 3 / 255, (-1.0, 1.0)),
+                        dict(sigma=3.0, block_step=5, bm_range=7, radius=1, ps_num=4, ps_range=3)),
     # predictor windows too wide for the row-parallel temporal search
     # (61 columns + 7): the per-wavefront search takes over
     "widest_predictive": (lambda s: s.shifted_clip(13, 3, 96, 80, 3 / 255, (1.0, 1.0)),
