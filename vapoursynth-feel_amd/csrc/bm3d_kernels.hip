@@ -705,7 +705,13 @@ __device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int n
     for (int i = 0; i < 8; ++i) {
         h[i] = ~0u;
     }
-    for (int r = sub; r < nrows; r += 16) {
+    if (sub < nrows) {
+        // the lane's first row: its keys are already sorted
+        h[0] = rt[3 * sub];
+        h[1] = rt[3 * sub + 1];
+        h[2] = rt[3 * sub + 2];
+    }
+    for (int r = sub + 16; r < nrows; r += 16) {
         const uint32_t qv[3] = {rt[3 * r], rt[3 * r + 1], rt[3 * r + 2]};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
