@@ -780,10 +780,11 @@ __device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int n
         }
         pos += int((own >> 1) & 1u);
     }
+    // (more than 16 staged: the whole-wavefront pass takes over, stop here)
     int staged = stored;
-    for (int q = 0; q < nslots; ++q) {
+    for (int q = 0; q < nslots && staged <= 16; ++q) {
         uint32_t seg = uint32_t(__ballot((scan >> q) & 1u) >> (grp * 16)) & 0xFFFFu;
-        while (seg) {
+        while (seg && staged <= 16) {
             const int r = q * 16 + __ffs(seg) - 1;
             seg &= seg - 1;
             const int rw = geo.len(r);
@@ -841,41 +842,51 @@ __device__ __forceinline__ void merge_staged(const Geo &geo, uint32_t thr, int s
 }
 
 // The same search by a whole wavefront, for more than 16 staged candidates
-// (ties): every row that may hold one (its first key's lower bound at most
-// thr) is scanned, the candidates staged in batches of 64, recomputed and
-// merged into the list.
+// (ties): the rows are scanned in seq order, the candidates that may be at
+// or below thr staged in batches of 64, recomputed and merged into the
+// list.  Once the list holds 8 candidates, a later one (larger seq) can only
+// enter with a distance below the list's 8th (lim, its bits): rows and
+// candidates whose lower bound is not below lim are skipped, which makes
+// windows of equal distances (flat or blank frames) cheap.
 template <class Geo>
 __device__ void select_wave(const uint32_t *rt, const uint16_t *db, int dcols, int nrows, const Geo &geo,
                             uint32_t thr, uint32_t pm, int lane, uint32_t *sxy, TopList &L)
 {
     list_reset(L);
     int staged = 0;
-    for (int r0 = 0; r0 < nrows; r0 += 64) {
-        bool any = false;
-        if (r0 + lane < nrows) {
-            any = (rt[3 * (r0 + lane)] & ~pm) <= thr;
+    uint32_t lim = kFltMaxBits;  // the list's 8th distance (placeholders: FLT_MAX)
+    for (int r = 0; r < nrows; ++r) {
+        const uint32_t lb = rt[3 * r] & ~pm;  // the row's lower bound
+        if (lb > thr || lb >= lim) {
+            continue;
         }
-        uint64_t rows = __ballot(any);
-        while (rows) {
-            const int r = r0 + __ffsll((unsigned long long)rows) - 1;
-            rows &= rows - 1;
-            const int rw = geo.len(r);
-            for (int k0 = 0; k0 < rw; k0 += 64) {
-                const int k = k0 + lane;
-                const bool maybe = k < rw && lower_bits(db[r * dcols + min(k, rw - 1)]) <= thr;
-                const uint64_t m = __ballot(maybe);
-                const int cnt = __popcll(m);
-                if (staged + cnt > 64) {
-                    merge_staged(geo, thr, staged, lane, sxy, L);
-                    staged = 0;
-                }
-                if (maybe) {
-                    const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
-                    sxy[p] = geo.xy(r, k);
-                }
-                staged += cnt;
+        const int rw = geo.len(r);
+        for (int k0 = 0; k0 < rw; k0 += 64) {
+            const int k = k0 + lane;
+            const uint32_t lo = k < rw ? lower_bits(db[r * dcols + min(k, rw - 1)]) : ~0u;
+            bool maybe = lo <= thr && lo < lim;
+            uint64_t m = __ballot(maybe);
+            int cnt = __popcll(m);
+            if (staged + cnt > 64) {
+                merge_staged(geo, thr, staged, lane, sxy, L);
+                staged = 0;
+                lim = uint32_t(list_threshold(L) >> 32);
+                maybe = maybe && lo < lim;
+                m = __ballot(maybe);
+                cnt = __popcll(m);
             }
+            if (maybe) {
+                const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
+                sxy[p] = geo.xy(r, k);
+            }
+            staged += cnt;
+        }
+        if (staged >= 16 && lim == kFltMaxBits) {
+            // enough for a full list: merge now to start skipping
+            merge_staged(geo, thr, staged, lane, sxy, L);
+            staged = 0;
+            lim = uint32_t(list_threshold(L) >> 32);
         }
     }
     merge_staged(geo, thr, staged, lane, sxy, L);
@@ -1180,8 +1191,10 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 // ---------------------------------------------------------------------------
 
 // the frame top-8 of half h when more than 16 candidates survive (ties): its
-// rows holding one are scanned by the whole wavefront, the survivors merged
-// into a list in batches of 64; returns it in lanes 0..7
+// rows holding one are scanned in seq order by the whole wavefront, the
+// survivors merged into a list in batches of 64; once the list holds 8, only
+// distances below its 8th can still enter (equal ones lose on seq), so the
+// rest of a window of equal distances is skipped; returns it in lanes 0..7
 __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint32_t thr_h, uint32_t pm,
                                        const float *tdist, int wh, int w, int rr, int rw, int left, int top,
                                        unsigned long long *sk, uint32_t *sxy, int lane)
@@ -1191,23 +1204,34 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
     list_reset(L);
     int staged = 0;
     uint32_t rows = uint32_t(__ballot((lane >> 5) == h && row_ok && (m.x & ~pm) <= thr_h) >> (h * 32));
+    uint32_t lim = kFltMaxBits;  // the list's 8th distance (placeholders: FLT_MAX)
+    auto merge = [&]() {
+        wave_lds_sync();
+        const uint64_t key = lane < staged ? sk[lane] : kNoKey;
+        const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
+        wave_lds_sync();
+        list_merge_batch(L, key, kxy, lane);
+        staged = 0;
+        lim = uint32_t(list_threshold(L) >> 32);
+    };
     while (rows) {
         const int src = h * 32 + __ffs(rows) - 1;
         rows &= rows - 1;
+        if ((uint32_t(__builtin_amdgcn_readlane(int(m.x), src)) & ~pm) >= lim) {
+            continue;
+        }
         const int srw = __builtin_amdgcn_readlane(rw, src), sl = __builtin_amdgcn_readlane(left, src);
         const int st = __builtin_amdgcn_readlane(top, src), sr = __builtin_amdgcn_readlane(rr, src);
         const int sw = __builtin_amdgcn_readlane(w, src);
         const float d = lane < srw ? tdist[src * wh + lane] : FLT_MAX;  // srw <= wh <= 32
-        const bool sv = lane < srw && survives(d, thr_h);
-        const uint64_t msk = __ballot(sv);
-        const int cnt = __popcll(msk);
+        bool sv = lane < srw && survives(d, thr_h) && __float_as_uint(d) < lim;
+        uint64_t msk = __ballot(sv);
+        int cnt = __popcll(msk);
         if (staged + cnt > 64) {
-            wave_lds_sync();
-            const uint64_t key = lane < staged ? sk[lane] : kNoKey;
-            const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
-            wave_lds_sync();
-            list_merge_batch(L, key, kxy, lane);
-            staged = 0;
+            merge();
+            sv = sv && __float_as_uint(d) < lim;
+            msk = __ballot(sv);
+            cnt = __popcll(msk);
         }
         if (sv) {
             const int p = staged + __builtin_amdgcn_mbcnt_hi(uint32_t(msk >> 32),
@@ -1216,6 +1240,9 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
             sxy[p] = uint32_t(sl + lane) | (uint32_t(st + sr) << 16);
         }
         staged += cnt;
+        if (staged >= 16 && lim == kFltMaxBits) {
+            merge();  // enough for a full list: merge now to start skipping
+        }
     }
     wave_lds_sync();
     const uint64_t key = lane < staged ? sk[lane] : kNoKey;
