@@ -1416,8 +1416,9 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         }
         // the scanned rows, lowest first in each half
         uint32_t rows = uint32_t(__ballot(scan) >> (half * 32));
-        while (__ballot(rows != 0u)) {
-            const bool has = rows != 0u;
+        // (a half past 16 staged goes to the fallback: it stops scanning)
+        while (__ballot(rows != 0u && staged <= 16)) {
+            const bool has = rows != 0u && staged <= 16;
             const int src = half * 32 + (has ? __ffs(rows) - 1 : 0);
             if (has) {
                 rows &= rows - 1u;
