@@ -33,7 +33,8 @@ class Clip:
 
     def __init__(self, planes: Sequence[np.ndarray], color_family: int, sample_type: int = FLOAT,
                  bits_per_sample: int = 32, sub_sampling_w: int = 0, sub_sampling_h: int = 0):
-        self._planes = [np.asarray(p) for p in planes]
+        # numpy arrays, or any plane with a shape and per-frame indexing
+        self._planes = [p if hasattr(p, "shape") and hasattr(p, "__getitem__") else np.asarray(p) for p in planes]
         self.color_family = color_family
         self.sample_type = sample_type
         self.bits_per_sample = bits_per_sample
