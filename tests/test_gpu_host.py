@@ -71,13 +71,17 @@ def test_yuv_chroma_passes_through_and_luma_is_denoised():
         assert np.abs(py - y[n]).max() > 1e-4
 
 
-def test_parallel_requests_are_deterministic():
+@pytest.mark.parametrize("radius,num_streams,nthreads", [(1, 4, 4), (0, 2, 6), (2, 3, 6)])
+def test_parallel_requests_are_deterministic(radius, num_streams, nthreads):
+    # requests overlap (uploads, kernels and downloads of different frames on
+    # separate streams); more threads than output slots make them wait for one
     _torch()
     from vsfeel import BM3Dv2, Clip, synth
     clip = synth.workload(3, frames=8, width=128, height=80)[0]
-    kw = dict(sigma=3.0, block_step=4, bm_range=8, radius=1, ps_num=2, ps_range=4, extractor_exp=2,
-              num_streams=4)
-    print(f"parallel requests clip={clip.shape}", flush=True)
+    kw = dict(sigma=3.0, block_step=4, bm_range=8, radius=radius, ps_num=2, ps_range=4, extractor_exp=2,
+              num_streams=num_streams)
+    print(f"parallel requests clip={clip.shape} radius={radius} streams={num_streams} threads={nthreads}",
+          flush=True)
     want = _device_frames(clip, {k: v for k, v in kw.items() if k != "num_streams"})
     node = BM3Dv2(Clip.gray(clip), **kw)
     got = {}
@@ -91,7 +95,7 @@ def test_parallel_requests_are_deterministic():
             errors.append(e)
 
     rng = np.random.default_rng(4)
-    threads = [threading.Thread(target=worker, args=(list(rng.permutation(8)),)) for _ in range(4)]
+    threads = [threading.Thread(target=worker, args=(list(rng.permutation(8)),)) for _ in range(nthreads)]
     for t in threads:
         t.start()
     for t in threads:
@@ -99,6 +103,6 @@ def test_parallel_requests_are_deterministic():
     assert not errors, errors
     assert sorted(got) == list(range(8))
     for n, frames in got.items():
-        assert len(frames) == 4
+        assert len(frames) == nthreads
         for f in frames:
             assert np.array_equal(f, want[n]), f"frame {n} differs under parallel requests"

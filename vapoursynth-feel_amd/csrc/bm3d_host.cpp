@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdint>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -175,31 +176,94 @@ struct Instance {
     GraphKey graph_key{};
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;  // orders the instance stream with a null-stream caller
 
-    // pinned host staging of the host-buffer path (allocated on first use):
-    // two upload planes used in turn, each with the event of its last copy,
-    // and one download plane; all with the library's plane pitch
+    // The host-buffer path (allocated on first use).  Requests overlap: a
+    // request enqueues its uploads on copy_in, its kernels on `stream` and
+    // its download on copy_out under the mutex, then waits for its download
+    // and copies the frame out without it, while later requests proceed.
+    // Each source ring slot has the event after the last kernels that read
+    // it, which an upload into the slot waits for; each request takes one of
+    // num_streams output slots (a device plane, a pinned download plane).
+    // Two pinned upload planes are used in turn, each with the event of its
+    // last copy.  All planes have the library's pitch.
+    hipStream_t copy_in = nullptr, copy_out = nullptr;
     float *pin_up[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     int pin_next = 0;
-    float *pin_down = nullptr;
+    hipEvent_t up_ev = nullptr;          // a request's uploads done (copy_in)
+    std::vector<hipEvent_t> slot_ev;     // per src ring slot: after its last readers (stream)
+    std::vector<char> slot_ev_set;
+    struct OutSlot {
+        float *dev = nullptr;            // the aggregated frame
+        float *pin = nullptr;            // its pinned copy
+        hipEvent_t done = nullptr;       // kernels done (stream)
+        hipEvent_t down = nullptr;       // download done (copy_out)
+        bool busy = false, used = false;
+    };
+    std::vector<OutSlot> outs;
+    int out_next = 0;
+    std::condition_variable out_cv;
 
     hipError_t pin_alloc()
     {
-        if (pin_down) {
+        if (!outs.empty()) {
             return hipSuccess;
         }
         const size_t bytes = size_t(pe) * sizeof(float);
-        hipError_t e = hipSuccess;
+        hipError_t e = hipStreamCreateWithFlags(&copy_in, hipStreamNonBlocking);
+        if (e == hipSuccess) {
+            e = hipStreamCreateWithFlags(&copy_out, hipStreamNonBlocking);
+        }
+        if (e == hipSuccess) {
+            e = hipEventCreateWithFlags(&up_ev, hipEventDisableTiming);
+        }
         for (int k = 0; k < 2 && e == hipSuccess; ++k) {
             e = hipHostMalloc(reinterpret_cast<void **>(&pin_up[k]), bytes, hipHostMallocDefault);
             if (e == hipSuccess) {
                 e = hipEventCreateWithFlags(&pin_ev[k], hipEventDisableTiming);
             }
         }
-        if (e == hipSuccess) {
-            e = hipHostMalloc(reinterpret_cast<void **>(&pin_down), bytes, hipHostMallocDefault);
+        slot_ev.assign(size_t(src_ring), nullptr);
+        slot_ev_set.assign(size_t(src_ring), 0);
+        for (int k = 0; k < src_ring && e == hipSuccess; ++k) {
+            e = hipEventCreateWithFlags(&slot_ev[k], hipEventDisableTiming);
+        }
+        std::vector<OutSlot> o(size_t(std::max(num_streams, 1)));
+        for (OutSlot &q : o) {
+            if (e == hipSuccess) {
+                e = hipMalloc(&q.dev, bytes);
+            }
+            if (e == hipSuccess) {
+                e = hipHostMalloc(reinterpret_cast<void **>(&q.pin), bytes, hipHostMallocDefault);
+            }
+            if (e == hipSuccess) {
+                e = hipEventCreateWithFlags(&q.done, hipEventDisableTiming);
+            }
+            if (e == hipSuccess) {
+                e = hipEventCreateWithFlags(&q.down, hipEventDisableTiming);
+            }
+        }
+        outs.swap(o);  // on failure the destructor frees what was made
+        if (e != hipSuccess) {
+            free_outs();
         }
         return e;
+    }
+
+    void free_outs()
+    {
+        for (OutSlot &q : outs) {
+            (void)hipFree(q.dev);
+            if (q.pin) {
+                (void)hipHostFree(q.pin);
+            }
+            if (q.done) {
+                (void)hipEventDestroy(q.done);
+            }
+            if (q.down) {
+                (void)hipEventDestroy(q.down);
+            }
+        }
+        outs.clear();
     }
 
     // copies a host plane (row stride in bytes) to a device plane of the
@@ -305,8 +369,10 @@ struct Instance {
             return;  // creation failed before any device object existed
         }
         (void)hipSetDevice(device);
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
+        for (hipStream_t q : {stream, copy_in, copy_out}) {
+            if (q) {
+                (void)hipStreamSynchronize(q);
+            }
         }
         (void)hipFree(src_buf);
         (void)hipFree(res_buf);
@@ -331,8 +397,19 @@ struct Instance {
                 (void)hipEventDestroy(pin_ev[k]);
             }
         }
-        if (pin_down) {
-            (void)hipHostFree(pin_down);
+        free_outs();
+        for (hipEvent_t e : slot_ev) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        if (up_ev) {
+            (void)hipEventDestroy(up_ev);
+        }
+        for (hipStream_t q : {copy_in, copy_out}) {
+            if (q) {
+                (void)hipStreamDestroy(q);
+            }
         }
         if (fork_ev) {
             (void)hipEventDestroy(fork_ev);
@@ -899,20 +976,41 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
     if (!clip_planes || !clip_strides || !dst || (d.final_pass && (!ref_planes || !ref_strides))) {
         return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing frame planes");
     }
-    std::lock_guard<std::mutex> lock(d.mu);
+    std::unique_lock<std::mutex> lock(d.mu);
     VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
     hipStream_t s = d.stream;
     VSFEEL_HIP(d.pin_alloc(), "hipHostMalloc");
+    // an output slot: the request owns it until its frame is copied out
+    d.out_cv.wait(lock, [&] { return !d.outs[size_t(d.out_next)].busy; });
+    const int j = d.out_next;
+    d.out_next = (d.out_next + 1) % int(d.outs.size());
+    Instance::OutSlot &o = d.outs[size_t(j)];
+    o.busy = true;
+    struct Release {
+        Instance &d;
+        Instance::OutSlot &o;
+        std::unique_lock<std::mutex> &lock;
+        ~Release()
+        {
+            if (!lock.owns_lock()) {
+                lock.lock();
+            }
+            o.busy = false;
+            d.out_cv.notify_all();
+        }
+    } release{d, o, lock};
+
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
     const int r = d.radius, nf = d.nframes;
     const int lo = clampi(n - 2 * r, 0, nf - 1), hi = clampi(n + 2 * r, 0, nf - 1);
     const size_t row_bytes = size_t(d.width) * sizeof(float);
 
-    // upload the window frames the ring does not hold yet (src/bm3d.cpp:772-814);
-    // at radius 0 the single slot is rewritten for every frame like the
-    // reference's per-stream slot
+    // upload the window frames the ring does not hold yet (src/bm3d.cpp:772-814)
+    // on copy_in, each after the kernels that last read its slot; at radius 0
+    // the frame is uploaded for every request (the reference's per-stream
+    // slot), into the next ring slot so that it overlaps earlier requests
     for (int f = lo; f <= hi; ++f) {
-        const int slot = r == 0 ? 0 : f % d.src_ring;
+        const int slot = f % d.src_ring;
         if (r > 0 && d.src_frame[slot] == f) {
             continue;
         }
@@ -920,13 +1018,19 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
             return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing plane of frame " + std::to_string(f));
         }
+        if (d.slot_ev_set[size_t(slot)]) {
+            VSFEEL_HIP(hipStreamWaitEvent(d.copy_in, d.slot_ev[size_t(slot)], 0), "hipStreamWaitEvent");
+        }
+        d.src_frame[slot] = -1;  // until the upload is enqueued
         // in the final pass the slot holds [ref][source] like the reference's ring
-        VSFEEL_HIP(d.upload(d.src_slot(slot, d.clips - 1), clip_planes[i], clip_strides[i], s), "upload");
+        VSFEEL_HIP(d.upload(d.src_slot(slot, d.clips - 1), clip_planes[i], clip_strides[i], d.copy_in), "upload");
         if (d.final_pass) {
-            VSFEEL_HIP(d.upload(d.src_slot(slot, 0), ref_planes[i], ref_strides[i], s), "upload");
+            VSFEEL_HIP(d.upload(d.src_slot(slot, 0), ref_planes[i], ref_strides[i], d.copy_in), "upload");
         }
         d.src_frame[slot] = f;
     }
+    VSFEEL_HIP(hipEventRecord(d.up_ev, d.copy_in), "hipEventRecord");
+    VSFEEL_HIP(hipStreamWaitEvent(s, d.up_ev, 0), "hipStreamWaitEvent");
 
     // estimate stacks of the window frames (src/bm3d.cpp:558-602)
     std::vector<int> slot_of(d.tw);
@@ -940,21 +1044,36 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         const float *search[kMaxWindow];
         const float *input[kMaxWindow];
         for (int z = 0; z < d.tw; ++z) {
-            const int fz = clampi(m - r + z, 0, nf - 1);
-            const int ring = r == 0 ? 0 : fz % d.src_ring;
+            const int ring = clampi(m - r + z, 0, nf - 1) % d.src_ring;
             search[z] = d.src_slot(ring, 0);
             input[z] = d.src_slot(ring, d.clips - 1);
         }
         VSFEEL_HIP(d.estimate(search, input, d.pitch, d.res_slot(slot), d.pitch, s), "estimate launch");
         d.res_frame[slot] = r == 0 ? -1 : m;
     }
-    VSFEEL_HIP(d.aggregate(n, slot_of, d.dst_dev, d.pitch, s), "aggregate launch");
-    VSFEEL_HIP(hipMemcpyAsync(d.pin_down, d.dst_dev, size_t(d.pe) * sizeof(float), hipMemcpyDeviceToHost, s),
-               "hipMemcpyAsync");
+    // the slot's previous frame may still be on its way out
+    if (o.used) {
+        VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+    }
+    VSFEEL_HIP(d.aggregate(n, slot_of, o.dev, d.pitch, s), "aggregate launch");
+    VSFEEL_HIP(hipEventRecord(o.done, s), "hipEventRecord");
+    for (int f = lo; f <= hi; ++f) {
+        const int slot = f % d.src_ring;
+        VSFEEL_HIP(hipEventRecord(d.slot_ev[size_t(slot)], s), "hipEventRecord");
+        d.slot_ev_set[size_t(slot)] = 1;
+    }
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
-    VSFEEL_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    VSFEEL_HIP(hipStreamWaitEvent(d.copy_out, o.done, 0), "hipStreamWaitEvent");
+    VSFEEL_HIP(hipMemcpyAsync(o.pin, o.dev, size_t(d.pe) * sizeof(float), hipMemcpyDeviceToHost, d.copy_out),
+               "hipMemcpyAsync");
+    VSFEEL_HIP(hipEventRecord(o.down, d.copy_out), "hipEventRecord");
+    o.used = true;
+
+    // the frame leaves the slot without the lock: later requests proceed
+    lock.unlock();
+    VSFEEL_HIP(hipEventSynchronize(o.down), "hipEventSynchronize");
     for (int y = 0; y < d.height; ++y) {
-        std::memcpy(reinterpret_cast<char *>(dst) + (ptrdiff_t)y * dst_stride, d.pin_down + (size_t)y * d.pitch,
+        std::memcpy(reinterpret_cast<char *>(dst) + (ptrdiff_t)y * dst_stride, o.pin + (size_t)y * d.pitch,
                     row_bytes);
     }
     return VSFEEL_BM3D_OK;
@@ -1046,9 +1165,10 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
         const Instance::GraphKey key{clip->data, clip->ref, clip->frame_stride, out_frame_stride,
                                      clip->pitch, first, count, out_pitch, out};
         if (!d.graph_exec || !(d.graph_key == key)) {
+            // the previous graph, or host-buffer requests, may still run on
+            // the instance stream: capture on an idle one
+            VSFEEL_HIP(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
             if (d.graph_exec) {
-                // the previous graph may still run on the instance stream
-                VSFEEL_HIP(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
                 (void)hipGraphExecDestroy(d.graph_exec);
                 d.graph_exec = nullptr;
             }
