@@ -23,12 +23,16 @@ forms (`forms`):
 * ``host_io``: the headline workload through the host-buffer entry point
   (the VapourSynth getFrame path): upload, filter and download per frame
   request, so PCIe-inclusive;
+* ``pipe``: the vspipe counterpart (``vsfeel.pipe_bench``) with the
+  reference harness's BM3Dv2 defaults (sigma 0.7, radius 2, bm_range 16,
+  ps_range 7, block_step 4): 1080p frames requested from 4 threads through
+  the host-buffer path;
 * ``c4``: 16 frames of 3840x2160 (radius 1, ps 2/7, sigma 5) in the FP32
   form, with frame 0 compared against the FP64 oracle (max-abs, PSNR, share
   of identical groups).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
-                    [--forms plain,fwd_bwd,temporal_r2,host_io,c4] [--cpu-seconds S]
+                    [--forms plain,fwd_bwd,temporal_r2,host_io,pipe,c4] [--cpu-seconds S]
                     [--copy-calibration-mb MB]
 """
 
@@ -235,7 +239,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--frames", type=int, default=None, help="override the clip length")
-    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2,host_io,c4",
+    ap.add_argument("--forms", default="plain,fwd_bwd,temporal_r2,host_io,pipe,c4",
                     help="comma list of forms to time (plain is the headline value)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-time budget of the CPU baseline sample (0: skip)")
@@ -278,6 +282,16 @@ def main():
                                                   "host buffers: upload, filter and download per "
                                                   "frame request, one thread (PCIe-inclusive)"))
         del node
+    if "pipe" in forms:
+        # the vspipe counterpart with the reference harness's BM3Dv2 defaults
+        # (benchmark/bench.py:167-180): 4 threads requesting 1080p frames
+        from vsfeel import pipe_bench
+        frames = 300
+        pipe_fps = pipe_bench.run(frames=frames, synthetic=False, threads=4)
+        extra["pipe"] = dict(value=pipe_fps, unit="frames/s",
+                             workload=f"{frames} frames of 1920x1080 GrayS float32 (16 synthetic frames cycled), "
+                                      "host buffers from 4 threads (PCIe-inclusive), sigma=0.7, block_step=4, "
+                                      "bm_range=16, radius=2, ps_range=7, num_streams=4")
     if "c4" in forms:
         extra["c4"] = bench_c4(max(1, args.steps // 2), 1, max(1, min(16, os.cpu_count() or 1)))
     if "temporal_r2" in forms:

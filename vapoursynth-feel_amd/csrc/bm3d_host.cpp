@@ -1000,15 +1000,25 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         }
     } release{d, o, lock};
 
+    // a request alone (every earlier one has copied its frame out, so their
+    // work is done) keeps its copies on the kernel stream: no cross-stream
+    // waits to pay when nothing can overlap
+    int in_flight = 0;
+    for (const Instance::OutSlot &q : d.outs) {
+        in_flight += q.busy ? 1 : 0;
+    }
+    const bool alone = in_flight == 1;
+    hipStream_t cin = alone ? s : d.copy_in, cout = alone ? s : d.copy_out;
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
     const int r = d.radius, nf = d.nframes;
     const int lo = clampi(n - 2 * r, 0, nf - 1), hi = clampi(n + 2 * r, 0, nf - 1);
     const size_t row_bytes = size_t(d.width) * sizeof(float);
 
     // upload the window frames the ring does not hold yet (src/bm3d.cpp:772-814)
-    // on copy_in, each after the kernels that last read its slot; at radius 0
-    // the frame is uploaded for every request (the reference's per-stream
-    // slot), into the next ring slot so that it overlaps earlier requests
+    // on the copy stream, each after the kernels that last read its slot; at
+    // radius 0 the frame is uploaded for every request (the reference's
+    // per-stream slot), into the next ring slot so that it overlaps earlier
+    // requests
     for (int f = lo; f <= hi; ++f) {
         const int slot = f % d.src_ring;
         if (r > 0 && d.src_frame[slot] == f) {
@@ -1018,19 +1028,21 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
             return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing plane of frame " + std::to_string(f));
         }
-        if (d.slot_ev_set[size_t(slot)]) {
-            VSFEEL_HIP(hipStreamWaitEvent(d.copy_in, d.slot_ev[size_t(slot)], 0), "hipStreamWaitEvent");
+        if (!alone && d.slot_ev_set[size_t(slot)]) {
+            VSFEEL_HIP(hipStreamWaitEvent(cin, d.slot_ev[size_t(slot)], 0), "hipStreamWaitEvent");
         }
         d.src_frame[slot] = -1;  // until the upload is enqueued
         // in the final pass the slot holds [ref][source] like the reference's ring
-        VSFEEL_HIP(d.upload(d.src_slot(slot, d.clips - 1), clip_planes[i], clip_strides[i], d.copy_in), "upload");
+        VSFEEL_HIP(d.upload(d.src_slot(slot, d.clips - 1), clip_planes[i], clip_strides[i], cin), "upload");
         if (d.final_pass) {
-            VSFEEL_HIP(d.upload(d.src_slot(slot, 0), ref_planes[i], ref_strides[i], d.copy_in), "upload");
+            VSFEEL_HIP(d.upload(d.src_slot(slot, 0), ref_planes[i], ref_strides[i], cin), "upload");
         }
         d.src_frame[slot] = f;
     }
-    VSFEEL_HIP(hipEventRecord(d.up_ev, d.copy_in), "hipEventRecord");
-    VSFEEL_HIP(hipStreamWaitEvent(s, d.up_ev, 0), "hipStreamWaitEvent");
+    if (!alone) {
+        VSFEEL_HIP(hipEventRecord(d.up_ev, cin), "hipEventRecord");
+        VSFEEL_HIP(hipStreamWaitEvent(s, d.up_ev, 0), "hipStreamWaitEvent");
+    }
 
     // estimate stacks of the window frames (src/bm3d.cpp:558-602)
     std::vector<int> slot_of(d.tw);
@@ -1052,7 +1064,7 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         d.res_frame[slot] = r == 0 ? -1 : m;
     }
     // the slot's previous frame may still be on its way out
-    if (o.used) {
+    if (o.used && !alone) {
         VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
     }
     VSFEEL_HIP(d.aggregate(n, slot_of, o.dev, d.pitch, s), "aggregate launch");
@@ -1063,10 +1075,12 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         d.slot_ev_set[size_t(slot)] = 1;
     }
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
-    VSFEEL_HIP(hipStreamWaitEvent(d.copy_out, o.done, 0), "hipStreamWaitEvent");
-    VSFEEL_HIP(hipMemcpyAsync(o.pin, o.dev, size_t(d.pe) * sizeof(float), hipMemcpyDeviceToHost, d.copy_out),
+    if (!alone) {
+        VSFEEL_HIP(hipStreamWaitEvent(cout, o.done, 0), "hipStreamWaitEvent");
+    }
+    VSFEEL_HIP(hipMemcpyAsync(o.pin, o.dev, size_t(d.pe) * sizeof(float), hipMemcpyDeviceToHost, cout),
                "hipMemcpyAsync");
-    VSFEEL_HIP(hipEventRecord(o.down, d.copy_out), "hipEventRecord");
+    VSFEEL_HIP(hipEventRecord(o.down, cout), "hipEventRecord");
     o.used = true;
 
     // the frame leaves the slot without the lock: later requests proceed
