@@ -1,11 +1,21 @@
 // BM3Dv2 kernels for gfx950 (CDNA4, wave64).
 //
-//   bm3d_match     block matching: spatial SSD search plus the temporal
-//                  predictive search, one wavefront per reference block
-//   bm3d_estimate  3-D DCT, hard threshold (or Wiener shrinkage), inverse,
-//                  weighted scatter into the estimate stack
-//   bm3d_aggregate per-pixel sum of the 2r+1 slices and the fp64 division
-//   fill_zero      clears estimate slots (a kernel, so it can sit in a graph)
+//   bm3d_match_rows       spatial search of the sliding metric (radius < 3),
+//                         a run of reference blocks per workgroup, a window
+//                         row per thread, threshold selection of the top-8
+//   bm3d_match_predictive temporal predictive search after it (radius 1-2),
+//                         one wavefront per reference block, both directions
+//   bm3d_match            block matching one wavefront per reference block:
+//                         the exact metric (radius 3-4) and temporal searches
+//                         with window sets too wide for the kernel above
+//   bm3d_estimate(_tiled) 3-D DCT, hard threshold (or Wiener shrinkage),
+//                         inverse, weighted scatter into the estimate stack
+//   bm3d_tile_reduce      the tiles' accumulators into the frame's own slice
+//   bm3d_weight_box       the other frames' weight planes from block origins
+//   bm3d_aggregate        per-pixel sum of the 2r+1 slices, fp64 division
+//   bm3d_divide           backward: upstream gradient / denominator
+//   fill_zero             clears estimate slots (a kernel, so it can sit in a
+//                         graph)
 //
 // The whole file is compiled with -ffp-contract=off: every multiply-add that
 // must be fused is written as __builtin_fmaf, everything else rounds twice,
@@ -1017,15 +1027,17 @@ __device__ __forceinline__ TopList select_predictive(const float *tdist, uint32_
 // Spatial search of the sliding metric, row-parallel.  A workgroup takes a
 // run of run_blocks consecutive ref blocks of one grid row; their windows
 // share the rows [top, bottom], so one source region (column-major in LDS)
-// serves them all.  Phase 1: every thread owns one (block, window row) and
-// walks the row: column SSDs v_j against reference column min(j, 7), then
+// serves them all, the reference blocks included.  Phase 1: every thread
+// owns one (block, window row) and walks the row: column SSDs v_j against
+// reference column min(j, 7), then
 // D(k) = ((v_k+v_k+1) + (v_k+2+v_k+3)) + ((v_k+4+v_k+5) + (v_k+6+v_k+7)),
-// written to the block's distance rows in LDS, while it keeps the row's 3
-// smallest distances.  Phase 2: one 16-lane row of a wavefront per block
-// takes the 8th smallest of those stored distances as a threshold (it bounds
-// the window's 8th smallest distance from above), gathers the candidates at
-// or below it (about 9 on natural images) and sorts them; blocks with more
-// than 16 such candidates go to a whole-wavefront pass.
+// whose upper halves go to the block's 16-bit distance rows in LDS, while it
+// keeps the row's 3 smallest row keys.  Phase 2: one 16-lane row of a
+// wavefront per block takes a threshold from the 8th smallest stored key (it
+// bounds the window's 8th smallest distance from above), gathers the
+// candidates that may be at or below it (about 9 on natural images),
+// recomputes their distances exactly and sorts them (select16); blocks with
+// more than 16 such candidates go to a whole-wavefront pass (select_wave).
 // ---------------------------------------------------------------------------
 template <int CM>
 __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
