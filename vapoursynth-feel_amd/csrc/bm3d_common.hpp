@@ -149,6 +149,10 @@ struct ReduceArgs {
     int width, height, res_pitch;
     int tiles_x, region_w, region_h;
     int add_existing;                // add to the slice instead of overwriting it
+    // radius 0: the frame itself instead of the slice (the aggregation of a
+    // single slice, float(double(wdst) / double(weight))), or nullptr
+    float *dst;
+    int dst_pitch;
 };
 
 // Aggregation of the 2r+1 slices that land on one output frame.

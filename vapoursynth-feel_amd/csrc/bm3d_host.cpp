@@ -467,9 +467,13 @@ struct Instance {
     // mode kFilterPlain / kFilterSave: block matching into the group table
     // (saved.groups when given), then filtering; kFilterApply: no matching,
     // the saved groups, masks and weights are applied to `input`
+    // fused_dst (radius 0, tiled, plain mode): the tile reduction writes the
+    // aggregated frame there instead of the slice; fuses() says when it does
+    bool fuses() const { return tiled && radius == 0; }
     hipError_t estimate(const float *const *search, const float *const *input, int in_pitch,
                         float *stack, int stack_pitch, hipStream_t s, float *dists_out = nullptr,
-                        int mode = kFilterPlain, const Saved &saved = Saved())
+                        int mode = kFilterPlain, const Saved &saved = Saved(), float *fused_dst = nullptr,
+                        int fused_pitch = 0)
     {
         hipError_t e = hipSuccess;
         if (!tiled || radius > 0 || short_windows) {
@@ -560,6 +564,10 @@ struct Instance {
         ra.region_w = region_w;
         ra.region_h = region_h;
         ra.add_existing = short_windows ? 1 : 0;
+        if (fused_dst && fuses()) {
+            ra.dst = fused_dst;
+            ra.dst_pitch = fused_pitch;
+        }
         return timed_launch(kKernelReduce, s, [&] { return launch_tile_reduce(ra, s); });
     }
 
@@ -1060,14 +1068,23 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
             search[z] = d.src_slot(ring, 0);
             input[z] = d.src_slot(ring, d.clips - 1);
         }
-        VSFEEL_HIP(d.estimate(search, input, d.pitch, d.res_slot(slot), d.pitch, s), "estimate launch");
+        // the slot's previous frame may still be on its way out (the
+        // fused reduction writes it)
+        if (o.used && !alone && d.fuses()) {
+            VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+        }
+        VSFEEL_HIP(d.estimate(search, input, d.pitch, d.res_slot(slot), d.pitch, s, nullptr, kFilterPlain,
+                              Instance::Saved(), o.dev, d.pitch),
+                   "estimate launch");
         d.res_frame[slot] = r == 0 ? -1 : m;
     }
-    // the slot's previous frame may still be on its way out
-    if (o.used && !alone) {
-        VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+    if (!d.fuses()) {
+        // the slot's previous frame may still be on its way out
+        if (o.used && !alone) {
+            VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+        }
+        VSFEEL_HIP(d.aggregate(n, slot_of, o.dev, d.pitch, s), "aggregate launch");
     }
-    VSFEEL_HIP(d.aggregate(n, slot_of, o.dev, d.pitch, s), "aggregate launch");
     VSFEEL_HIP(hipEventRecord(o.done, s), "hipEventRecord");
     for (int f = lo; f <= hi; ++f) {
         const int slot = f % d.src_ring;
@@ -1160,15 +1177,20 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
                 const float *search[kMaxWindow];
                 const float *input[kMaxWindow];
                 device_window(d, clip, m, search, input);
-                hipError_t e = d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, q);
+                // radius 0: the tile reduction writes the frame (fused aggregation)
+                hipError_t e = d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, q, nullptr,
+                                          kFilterPlain, Instance::Saved(),
+                                          out + (int64_t)(n - first) * out_frame_stride, out_pitch);
                 if (e != hipSuccess) {
                     return e;
                 }
                 held[slot] = m;
             }
-            hipError_t e = d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, q);
-            if (e != hipSuccess) {
-                return e;
+            if (!d.fuses()) {
+                hipError_t e = d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, q);
+                if (e != hipSuccess) {
+                    return e;
+                }
             }
         }
         return hipSuccess;

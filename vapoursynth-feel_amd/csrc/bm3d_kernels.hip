@@ -2109,6 +2109,11 @@ __global__ __launch_bounds__(256) void bm3d_tile_reduce(ReduceArgs A)
         wd += A.res[q];
         wt += A.res[qw];
     }
+    if (A.dst) {
+        // radius 0: bm3d_aggregate of this single slice, fused
+        A.dst[(long long)y * A.dst_pitch + x] = (float)((double)wd / (double)wt);
+        return;
+    }
     A.res[q] = wd;
     A.res[qw] = wt;
 }
