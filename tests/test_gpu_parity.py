@@ -69,6 +69,15 @@ CASES = {
     # four predictors per step (two staging passes), 8 group entries
     "four_predictors": (lambda s: s.shifted_clip(15, 4, 88, 72, 3 / 255, (-1.0, 1.0)),
                         dict(sigma=3.0, block_step=5, bm_range=7, radius=1, ps_num=4, ps_range=3)),
+    # texture constant along rows: a block's best matches crowd into its own
+    # row and the rows a period away, past the first threshold's 16 slots
+    # (frame 7 of the 1080p generator, at a smaller size, and pure stripes)
+    "row_texture": (lambda s: s.workload(2, frames=8, width=320, height=192)[6:],
+                    dict(sigma=3.0, block_step=4, bm_range=16, radius=0)),
+    "stripes": (lambda s: np.clip(0.5 + 0.1 * np.sin(2 * np.pi * np.arange(96) / 9.0)[None, :, None]
+                                  + np.random.default_rng(16).normal(0.0, 3 / 255, (2, 96, 160)),
+                                  0.0, 1.0).astype(np.float32),
+                dict(sigma=3.0, block_step=4, bm_range=12, radius=1, ps_num=2, ps_range=4)),
     # predictor windows too wide for the row-parallel temporal search
     # (61 columns + 7): the per-wavefront search takes over
     "widest_predictive": (lambda s: s.shifted_clip(13, 3, 96, 80, 3 / 255, (1.0, 1.0)),

@@ -685,55 +685,23 @@ __device__ __forceinline__ bool survives(float d, uint32_t thr)
     return b <= thr && b < kFltMaxBits;
 }
 
-// ---------------------------------------------------------------------------
-// Top-8 (distance, seq) keys of one search by a 16-lane row of the wavefront
-// (sub = lane & 15), from the 3 smallest row keys (row_key, mask pm) of each
-// of its nrows rows (rt, 3 words per row) and its distance rows (db: upper
-// halves of the distances, row stride dcols).  The 8th smallest stored key
-// tk, ties counted, belongs to 8 distinct candidates whose distance bits are
-// at most tk | pm, so the threshold thr = tk | pm bounds the search's 8th
-// smallest distance from above.  Each lane keeps the sorted 8 smallest keys
-// of its rows (sub, sub+16, ...) and the row takes the 8 smallest heads in
-// turn.  The candidates whose distance bits may be at most thr are gathered
-// into the row's 16-entry stage (their positions): a row whose third key has
-// lower bound above thr holds them among its first two keys, the other rows
-// (usually the best match's row and a neighbour) are scanned by the lower
-// bounds of their distances.  Each lane then recomputes its staged
-// candidate's distance exactly and keeps it if it is at most thr.  Returns
-// the number staged and sets thr; when it is at most 16, lanes 0..7 of the
-// row hold the sorted top-8 in key/kxy (placeholders where the search has
-// fewer than 8 candidates).  nrows is the same for all rows of the wavefront.
-// ---------------------------------------------------------------------------
-template <class Geo>
-__device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int nrows, const Geo &geo, bool act,
-                        int lane, uint32_t *sxy, uint64_t &key, uint32_t &kxy, uint32_t &thr, uint32_t pm)
+// ascending 8-entry list of one lane: insert v (the largest drops out)
+__device__ __forceinline__ void list8_insert(uint32_t (&h)[8], uint32_t v)
 {
-    const int grp = lane >> 4, sub = lane & 15;
-    const int nslots = (nrows + 15) >> 4;
-    uint32_t h[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        h[i] = ~0u;
+        const uint32_t lo = min(h[i], v);
+        v = max(h[i], v);
+        h[i] = lo;
     }
-    if (sub < nrows) {
-        // the lane's first row: its keys are already sorted
-        h[0] = rt[3 * sub];
-        h[1] = rt[3 * sub + 1];
-        h[2] = rt[3 * sub + 2];
-    }
-    for (int r = sub + 16; r < nrows; r += 16) {
-        const uint32_t qv[3] = {rt[3 * r], rt[3 * r + 1], rt[3 * r + 2]};
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            uint32_t v = qv[j];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const uint32_t lo = min(h[i], v);
-                v = max(h[i], v);
-                h[i] = lo;
-            }
-        }
-    }
+}
+
+// The 8th smallest of the values a 16-lane row holds in its lanes' ascending
+// lists h, ties counted: the smallest m with at least 8 values <= m (~0u when
+// the row holds fewer than 8).  Consumes the lists.
+__device__ __forceinline__ uint32_t row_eighth(uint32_t (&h)[8], int lane)
+{
+    const int grp = lane >> 4;
     uint32_t tk = ~0u;
     int taken = 0;
 #pragma unroll
@@ -753,60 +721,149 @@ __device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int n
             h[7] = ~0u;
         }
     }
-    thr = tk | pm;
-    // stored candidates (2 bits per row slot) and rows to scan (1 bit); the
-    // row's candidates past its first two keys have distance bits at least
-    // (third key) & ~pm.  Where a row is not scanned thr < ~pm, so keys
-    // passing the bound are real candidates.
-    uint32_t own = 0, scan = 0;
-    int cnt = 0;
-    for (int q = 0; q < nslots; ++q) {
-        const int r = q * 16 + sub;
-        if (act && r < nrows) {
-            if ((rt[3 * r + 2] & ~pm) <= thr) {
-                scan |= 1u << q;
-            } else {
-                const uint32_t s0 = (rt[3 * r] & ~pm) <= thr ? 1u : 0u;
-                const uint32_t s1 = (rt[3 * r + 1] & ~pm) <= thr ? 1u : 0u;
-                own |= (s0 | (s1 << 1)) << (2 * q);
-                cnt += int(s0 + s1);
+    return tk;
+}
+
+// ---------------------------------------------------------------------------
+// Top-8 (distance, seq) keys of one search by a 16-lane row of the wavefront
+// (sub = lane & 15), from the 3 smallest row keys (row_key, mask pm) of each
+// of its nrows rows (rt, 3 words per row) and its distance rows (db: upper
+// halves of the distances, row stride dcols).  The 8th smallest stored key
+// tk, ties counted, belongs to 8 distinct candidates whose distance bits are
+// at most tk | pm, so the threshold thr = tk | pm bounds the search's 8th
+// smallest distance from above.  Each lane keeps the sorted 8 smallest keys
+// of its rows (sub, sub+16, ...) and the row takes the 8 smallest heads in
+// turn.  The candidates whose distance bits may be at most thr are gathered
+// into the row's 16-entry stage (their positions): a row whose third key has
+// lower bound above thr holds them among its first two keys, the other rows
+// (usually the best match's row and a neighbour) are scanned by the lower
+// bounds of their distances.  Where more than 16 are gathered (the best
+// matches crowd into the scanned rows, as on texture that is constant along
+// rows), thr is lowered to the 8th smallest upper bound of the scanned rows'
+// distances (stored upper half, lower half all ones), again a bound of 8
+// distinct candidates, and the candidates are gathered again.  Each lane
+// then recomputes its staged candidate's distance exactly and keeps it if it
+// is at most thr.  Returns the number staged and sets thr; when it is at most
+// 16, lanes 0..7 of the row hold the sorted top-8 in key/kxy (placeholders
+// where the search has fewer than 8 candidates).  nrows is the same for all
+// rows of the wavefront.
+// ---------------------------------------------------------------------------
+template <class Geo>
+__device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int nrows, const Geo &geo, bool act,
+                        int lane, uint32_t *sxy, uint64_t &key, uint32_t &kxy, uint32_t &thr, uint32_t pm)
+{
+    const int grp = lane >> 4, sub = lane & 15;
+    const int nslots = (nrows + 15) >> 4;
+    uint32_t h[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        h[i] = ~0u;
+    }
+    if (sub < nrows) {
+        // the lane's first row: its keys are already sorted
+        h[0] = rt[3 * sub];
+        h[1] = rt[3 * sub + 1];
+        h[2] = rt[3 * sub + 2];
+    }
+    for (int r = sub + 16; r < nrows; r += 16) {
+        list8_insert(h, rt[3 * r]);
+        list8_insert(h, rt[3 * r + 1]);
+        list8_insert(h, rt[3 * r + 2]);
+    }
+    thr = row_eighth(h, lane) | pm;
+
+    // Gathers the candidates that may be at or below thr (their positions,
+    // the first 16 in sxy); returns their number and the rows scanned (bit q:
+    // row q * 16 + sub).  Stored candidates: 2 bits per row slot.  The row's
+    // candidates past its first two keys have distance bits at least (third
+    // key) & ~pm.  Where a row is not scanned thr < ~pm, so keys passing the
+    // bound are real candidates.  Stops early once more than 16 are found.
+    auto gather = [&](uint32_t &scan) -> int {
+        uint32_t own = 0;
+        scan = 0;
+        int cnt = 0;
+        for (int q = 0; q < nslots; ++q) {
+            const int r = q * 16 + sub;
+            if (act && r < nrows) {
+                if ((rt[3 * r + 2] & ~pm) <= thr) {
+                    scan |= 1u << q;
+                } else {
+                    const uint32_t s0 = (rt[3 * r] & ~pm) <= thr ? 1u : 0u;
+                    const uint32_t s1 = (rt[3 * r + 1] & ~pm) <= thr ? 1u : 0u;
+                    own |= (s0 | (s1 << 1)) << (2 * q);
+                    cnt += int(s0 + s1);
+                }
             }
         }
-    }
-    const int incl = row_prefix_sum16(cnt);
-    int pos = incl - cnt;
-    const int stored = __shfl(incl, (lane & ~15) | 15);
-    for (int q = 0; own != 0; ++q, own >>= 2) {
-        if ((own & 3u) == 0) {
-            continue;
+        const int incl = row_prefix_sum16(cnt);
+        int pos = incl - cnt;
+        const int stored = __shfl(incl, (lane & ~15) | 15);
+        for (int q = 0; own != 0; ++q, own >>= 2) {
+            if ((own & 3u) == 0) {
+                continue;
+            }
+            const int r = q * 16 + sub;
+            if ((own & 1u) && pos < 16) {
+                sxy[pos] = geo.xy(r, int(rt[3 * r] & pm));
+            }
+            pos += int(own & 1u);
+            if ((own & 2u) && pos < 16) {
+                sxy[pos] = geo.xy(r, int(rt[3 * r + 1] & pm));
+            }
+            pos += int((own >> 1) & 1u);
         }
-        const int r = q * 16 + sub;
-        if ((own & 1u) && pos < 16) {
-            sxy[pos] = geo.xy(r, int(rt[3 * r] & pm));
-        }
-        pos += int(own & 1u);
-        if ((own & 2u) && pos < 16) {
-            sxy[pos] = geo.xy(r, int(rt[3 * r + 1] & pm));
-        }
-        pos += int((own >> 1) & 1u);
-    }
-    // (more than 16 staged: the whole-wavefront pass takes over, stop here)
-    int staged = stored;
-    for (int q = 0; q < nslots && staged <= 16; ++q) {
-        uint32_t seg = uint32_t(__ballot((scan >> q) & 1u) >> (grp * 16)) & 0xFFFFu;
-        while (seg && staged <= 16) {
-            const int r = q * 16 + __ffs(seg) - 1;
-            seg &= seg - 1;
-            const int rw = geo.len(r);
-            for (int k0 = 0; k0 < rw; k0 += 16) {
-                const int k = k0 + sub;
-                const bool maybe = k < rw && lower_bits(db[r * dcols + min(k, rw - 1)]) <= thr;
-                const uint32_t sm = uint32_t(__ballot(maybe) >> (grp * 16)) & 0xFFFFu;
-                const int p = staged + __popc(sm & ((1u << sub) - 1u));
-                if (maybe && p < 16) {
-                    sxy[p] = geo.xy(r, k);
+        int staged = stored;
+        for (int q = 0; q < nslots && staged <= 16; ++q) {
+            uint32_t seg = uint32_t(__ballot((scan >> q) & 1u) >> (grp * 16)) & 0xFFFFu;
+            while (seg && staged <= 16) {
+                const int r = q * 16 + __ffs(seg) - 1;
+                seg &= seg - 1;
+                const int rw = geo.len(r);
+                for (int k0 = 0; k0 < rw; k0 += 16) {
+                    const int k = k0 + sub;
+                    const bool maybe = k < rw && lower_bits(db[r * dcols + min(k, rw - 1)]) <= thr;
+                    const uint32_t sm = uint32_t(__ballot(maybe) >> (grp * 16)) & 0xFFFFu;
+                    const int p = staged + __popc(sm & ((1u << sub) - 1u));
+                    if (maybe && p < 16) {
+                        sxy[p] = geo.xy(r, k);
+                    }
+                    staged += __popc(sm);
                 }
-                staged += __popc(sm);
+            }
+        }
+        return staged;
+    };
+    uint32_t scan;
+    int staged = gather(scan);
+    if (__ballot(staged > 16) != 0) {
+        // the 8th smallest upper bound of the scanned rows' distances
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            h[i] = ~0u;
+        }
+        for (int q = 0; q < nslots; ++q) {
+            uint32_t seg = uint32_t(__ballot(staged > 16 && ((scan >> q) & 1u)) >> (grp * 16)) & 0xFFFFu;
+            while (seg) {
+                const int r = q * 16 + __ffs(seg) - 1;
+                seg &= seg - 1;
+                const int rw = geo.len(r);
+                for (int k0 = 0; k0 < rw; k0 += 16) {
+                    const int k = k0 + sub;
+                    if (k < rw) {
+                        list8_insert(h, lower_bits(db[r * dcols + k]) | 0xFFFFu);
+                    }
+                }
+            }
+        }
+        const uint32_t t = row_eighth(h, lane);
+        const bool tighter = staged > 16 && t < thr;
+        if (__ballot(tighter) != 0) {
+            if (tighter) {
+                thr = t;
+            }
+            const int again = gather(scan);
+            if (tighter) {
+                staged = again;
             }
         }
     }
