@@ -835,14 +835,17 @@ __device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int n
     };
     uint32_t scan;
     int staged = gather(scan);
-    if (__ballot(staged > 16) != 0) {
+    // (an upper bound below thr needs thr above 0xFFFF; equal distances, as
+    // on flat content, never get past it)
+    if (__ballot(staged > 16 && thr > 0xFFFFu) != 0) {
         // the 8th smallest upper bound of the scanned rows' distances
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             h[i] = ~0u;
         }
         for (int q = 0; q < nslots; ++q) {
-            uint32_t seg = uint32_t(__ballot(staged > 16 && ((scan >> q) & 1u)) >> (grp * 16)) & 0xFFFFu;
+            uint32_t seg =
+                uint32_t(__ballot(staged > 16 && thr > 0xFFFFu && ((scan >> q) & 1u)) >> (grp * 16)) & 0xFFFFu;
             while (seg) {
                 const int r = q * 16 + __ffs(seg) - 1;
                 seg &= seg - 1;
@@ -856,7 +859,7 @@ __device__ int select16(const uint32_t *rt, const uint16_t *db, int dcols, int n
             }
         }
         const uint32_t t = row_eighth(h, lane);
-        const bool tighter = staged > 16 && t < thr;
+        const bool tighter = staged > 16 && thr > 0xFFFFu && t < thr;
         if (__ballot(tighter) != 0) {
             if (tighter) {
                 thr = t;
@@ -914,7 +917,8 @@ __device__ __forceinline__ void merge_staged(const Geo &geo, uint32_t thr, int s
 // list.  Once the list holds 8 candidates, a later one (larger seq) can only
 // enter with a distance below the list's 8th (lim, its bits): rows and
 // candidates whose lower bound is not below lim are skipped, which makes
-// windows of equal distances (flat or blank frames) cheap.
+// windows of equal distances (flat or blank frames) cheap.  Needs nrows <=
+// 64 (lane r holds row r's lower bound).
 template <class Geo>
 __device__ void select_wave(const uint32_t *rt, const uint16_t *db, int dcols, int nrows, const Geo &geo,
                             uint32_t thr, uint32_t pm, int lane, uint32_t *sxy, TopList &L)
@@ -922,11 +926,11 @@ __device__ void select_wave(const uint32_t *rt, const uint16_t *db, int dcols, i
     list_reset(L);
     int staged = 0;
     uint32_t lim = kFltMaxBits;  // the list's 8th distance (placeholders: FLT_MAX)
-    for (int r = 0; r < nrows; ++r) {
-        const uint32_t lb = rt[3 * r] & ~pm;  // the row's lower bound
-        if (lb > thr || lb >= lim) {
-            continue;
-        }
+    const uint32_t rlb = lane < nrows ? rt[3 * lane] & ~pm : ~0u;  // row lane's lower bound
+    uint64_t rows = __ballot(lane < nrows && rlb <= thr);
+    while (rows != 0) {
+        const int r = __ffsll((unsigned long long)rows) - 1;
+        rows &= rows - 1;
         const int rw = geo.len(r);
         for (int k0 = 0; k0 < rw; k0 += 64) {
             const int k = k0 + lane;
@@ -938,6 +942,7 @@ __device__ void select_wave(const uint32_t *rt, const uint16_t *db, int dcols, i
                 merge_staged(geo, thr, staged, lane, sxy, L);
                 staged = 0;
                 lim = uint32_t(list_threshold(L) >> 32);
+                rows &= __ballot(rlb < lim);
                 maybe = maybe && lo < lim;
                 m = __ballot(maybe);
                 cnt = __popcll(m);
@@ -954,9 +959,12 @@ __device__ void select_wave(const uint32_t *rt, const uint16_t *db, int dcols, i
             merge_staged(geo, thr, staged, lane, sxy, L);
             staged = 0;
             lim = uint32_t(list_threshold(L) >> 32);
+            rows &= __ballot(rlb < lim);
         }
     }
-    merge_staged(geo, thr, staged, lane, sxy, L);
+    if (staged > 0) {
+        merge_staged(geo, thr, staged, lane, sxy, L);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1313,11 +1321,9 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
             merge();  // enough for a full list: merge now to start skipping
         }
     }
-    wave_lds_sync();
-    const uint64_t key = lane < staged ? sk[lane] : kNoKey;
-    const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
-    wave_lds_sync();
-    list_merge_batch(L, key, kxy, lane);
+    if (staged > 0) {
+        merge();
+    }
     return L;
 }
 
