@@ -149,6 +149,7 @@ struct ReduceArgs {
     int width, height, res_pitch;
     int tiles_x, region_w, region_h;
     int add_existing;                // add to the slice instead of overwriting it
+    int span3;                       // every pixel lies in at most 3 x 3 tiles, scratch < 2^32 elements
     // radius 0: the frame itself instead of the slice (the aggregation of a
     // single slice, float(double(wdst) / double(weight))), or nullptr
     float *dst;

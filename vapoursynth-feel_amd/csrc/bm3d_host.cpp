@@ -149,6 +149,7 @@ struct Instance {
     float *scratch = nullptr;    // tiles * 2 * region_h * region_w
     float *origins = nullptr;    // radius > 0: 2r origin weight planes of height * pitch
     int *tile_tables = nullptr;  // col_tiles[W] (int2), row_tiles[H] (int2), tile_x0[tiles_x], tile_y0[tiles_y]
+    bool reduce_span3 = false;   // every pixel lies in at most 3 x 3 tiles (bm3d_tile_reduce's unrolled form)
     hipStream_t stream = nullptr;
     hipEvent_t last_done = nullptr;  // end of the previous device-side call
     bool have_last = false;
@@ -564,6 +565,7 @@ struct Instance {
         ra.region_w = region_w;
         ra.region_h = region_h;
         ra.add_existing = short_windows ? 1 : 0;
+        ra.span3 = reduce_span3 ? 1 : 0;
         if (fused_dst && fuses()) {
             ra.dst = fused_dst;
             ra.dst_pitch = fused_pitch;
@@ -909,6 +911,14 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         std::copy(x_lo.begin(), x_lo.end(), tab.begin() + 2 * d.width + 2 * d.height);
         std::copy(y_lo.begin(), y_lo.end(), tab.begin() + 2 * d.width + 2 * d.height + d.tiles_x);
         const size_t scratch_elems = size_t(d.tiles_x) * d.tiles_y * 2 * d.region_h * d.region_w;
+        int span_x = 0, span_y = 0;
+        for (int p = 0; p < d.width; ++p) {
+            span_x = std::max(span_x, tab[2 * p + 1] - tab[2 * p] + 1);
+        }
+        for (int p = 0; p < d.height; ++p) {
+            span_y = std::max(span_y, tab[2 * d.width + 2 * p + 1] - tab[2 * d.width + 2 * p] + 1);
+        }
+        d.reduce_span3 = span_x <= 3 && span_y <= 3 && scratch_elems < (size_t(1) << 32);
         if ((e = hipMalloc(&d.scratch, scratch_elems * sizeof(float))) != hipSuccess ||
             (e = hipMalloc(&d.tile_tables, tab.size() * sizeof(int))) != hipSuccess) {
             return dev_fail(e, "hipMalloc");

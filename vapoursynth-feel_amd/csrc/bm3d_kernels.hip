@@ -2144,7 +2144,9 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
 
 // Dense (wdst, weight) planes of the reference frame's own slice from the
 // tile accumulators: every pixel adds the tiles whose region covers it, in
-// tile order.
+// tile order.  Where every pixel lies in at most 3 x 3 tiles (span3) all of
+// a pixel's loads are issued before the first add (absent tiles add +0.0f,
+// which leaves the sum's bits unchanged), with 32-bit offsets.
 __global__ __launch_bounds__(256) void bm3d_tile_reduce(ReduceArgs A)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -2154,15 +2156,41 @@ __global__ __launch_bounds__(256) void bm3d_tile_reduce(ReduceArgs A)
     }
     const int2 cr = A.col_tiles[x];
     const int2 rr = A.row_tiles[y];
-    const long long plane = (long long)A.region_h * A.region_w;
     float wd = 0.0f, wt = 0.0f;
-    for (int ty = rr.x; ty <= rr.y; ++ty) {
-        const int ly = y - A.tile_y0[ty];
-        for (int tx = cr.x; tx <= cr.y; ++tx) {
-            const int lx = x - A.tile_x0[tx];
-            const float *t = A.scratch + (long long)(ty * A.tiles_x + tx) * 2 * plane + (long long)ly * A.region_w + lx;
-            wd += t[0];
-            wt += t[plane];
+    if (A.span3) {
+        const uint32_t plane = uint32_t(A.region_h) * uint32_t(A.region_w);
+        const int nty = rr.y - rr.x + 1, ntx = cr.y - cr.x + 1;
+        float vd[9], vw[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int ty = rr.x + max(min(i, nty - 1), 0);
+            const int ly = min(max(y - A.tile_y0[ty], 0), A.region_h - 1);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int tx = cr.x + max(min(j, ntx - 1), 0);
+                const int lx = min(max(x - A.tile_x0[tx], 0), A.region_w - 1);
+                const uint32_t off = uint32_t(ty * A.tiles_x + tx) * 2u * plane + uint32_t(ly * A.region_w + lx);
+                const bool ok = i < nty && j < ntx;
+                vd[3 * i + j] = ok ? A.scratch[off] : 0.0f;
+                vw[3 * i + j] = ok ? A.scratch[off + plane] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            wd += vd[k];
+            wt += vw[k];
+        }
+    } else {
+        const long long plane = (long long)A.region_h * A.region_w;
+        for (int ty = rr.x; ty <= rr.y; ++ty) {
+            const int ly = y - A.tile_y0[ty];
+            for (int tx = cr.x; tx <= cr.y; ++tx) {
+                const int lx = x - A.tile_x0[tx];
+                const float *t =
+                    A.scratch + (long long)(ty * A.tiles_x + tx) * 2 * plane + (long long)ly * A.region_w + lx;
+                wd += t[0];
+                wt += t[plane];
+            }
         }
     }
     const long long q = (long long)y * A.res_pitch + x;
