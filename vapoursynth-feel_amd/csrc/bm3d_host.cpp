@@ -29,6 +29,7 @@ hipError_t launch_aggregate(const AggArgs &a, hipStream_t s);
 hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
 hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s);
 hipError_t launch_weight_box(const BoxArgs &a, hipStream_t s);
+hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
 
 namespace {
 
@@ -68,6 +69,7 @@ enum KernelKind {
     kKernelAggregate,
     kKernelReduce,
     kKernelWeightBox,
+    kKernelGather,
     kKernelKinds
 };
 
@@ -150,6 +152,12 @@ struct Instance {
     float *origins = nullptr;    // radius > 0: 2r origin weight planes of height * pitch
     int *tile_tables = nullptr;  // col_tiles[W] (int2), row_tiles[H] (int2), tile_x0[tiles_x], tile_y0[tiles_y]
     bool reduce_span3 = false;   // every pixel lies in at most 3 x 3 tiles (bm3d_tile_reduce's unrolled form)
+    // radius > 0, tiled: the other frames' blocks binned by 64x64 cell (see
+    // EstimateArgs::rec), summed by bm3d_gather
+    float *rec = nullptr;
+    uint32_t *rec_pos = nullptr, *cell_count = nullptr, *cell_list = nullptr;
+    int rec_per_tile = 0, cells_x = 0, ncells = 0, cell_cap = 0;
+    long long cell_count_elems = 0;  // cleared per estimate (a multiple of 64)
     hipStream_t stream = nullptr;
     hipEvent_t last_done = nullptr;  // end of the previous device-side call
     bool have_last = false;
@@ -384,6 +392,10 @@ struct Instance {
         (void)hipFree(scratch);
         (void)hipFree(origins);
         (void)hipFree(tile_tables);
+        (void)hipFree(rec);
+        (void)hipFree(rec_pos);
+        (void)hipFree(cell_count);
+        (void)hipFree(cell_list);
         if (last_done) {
             (void)hipEventDestroy(last_done);
         }
@@ -495,6 +507,14 @@ struct Instance {
                 return e;
             }
         }
+        if (rec) {
+            e = timed_launch(kKernelFill, s, [&] {
+                return launch_fill_zero(reinterpret_cast<float *>(cell_count), cell_count_elems, s);
+            });
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
         uint32_t *table = saved.groups ? saved.groups : groups;
         if (mode != kFilterApply) {
             MatchArgs m = match_args(search, in_pitch, dists_out);
@@ -535,8 +555,32 @@ struct Instance {
             a.tiles_y = tiles_y;
             a.region_w = region_w;
             a.region_h = region_h;
+            a.rec = rec;
+            a.rec_pos = rec_pos;
+            a.cell_count = cell_count;
+            a.cell_list = cell_list;
+            a.rec_per_tile = rec_per_tile;
+            a.cells_x = cells_x;
+            a.ncells = ncells;
+            a.cell_cap = cell_cap;
         }
         e = timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, mode, s); });
+        if (e == hipSuccess && rec && tiled) {
+            GatherArgs ga {};
+            ga.rec = rec;
+            ga.rec_pos = rec_pos;
+            ga.cell_count = cell_count;
+            ga.cell_list = cell_list;
+            ga.res = stack;
+            ga.width = width;
+            ga.height = height;
+            ga.res_pitch = stack_pitch;
+            ga.cells_x = cells_x;
+            ga.ncells = ncells;
+            ga.cell_cap = cell_cap;
+            ga.radius = radius;
+            e = timed_launch(kKernelGather, s, [&] { return launch_gather(ga, s); });
+        }
         if (e == hipSuccess && origins) {
             BoxArgs b {};
             b.origins = origins;
@@ -926,6 +970,30 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         if ((e = hipMemcpy(d.tile_tables, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice)) !=
             hipSuccess) {
             return dev_fail(e, "hipMemcpy");
+        }
+        if (d.radius > 0) {
+            // a group holds at most 7 other-frame blocks (the self block is
+            // always a member); binning is skipped where its records would
+            // take more than 4 GiB
+            const long long tiles = (long long)d.tiles_x * d.tiles_y;
+            const long long per_tile = (long long)d.tile_bx * d.tile_by * 7;
+            const long long rec_bytes = tiles * per_tile * 260;
+            const int cw = 1 << kCellShift;
+            const int cx = (d.width + cw - 1) / cw, cy = (d.height + cw - 1) / cw;
+            if (rec_bytes <= (4LL << 30) && tiles * per_tile < (1LL << 32)) {
+                d.rec_per_tile = int(per_tile);
+                d.cells_x = cx;
+                d.ncells = cx * cy;
+                d.cell_cap = 512;
+                d.cell_count_elems = (2LL * d.radius * d.ncells + 63) / 64 * 64;
+                if ((e = hipMalloc(&d.rec, size_t(tiles * per_tile) * 64 * sizeof(float))) != hipSuccess ||
+                    (e = hipMalloc(&d.rec_pos, size_t(tiles * per_tile) * sizeof(uint32_t))) != hipSuccess ||
+                    (e = hipMalloc(&d.cell_count, size_t(d.cell_count_elems) * sizeof(uint32_t))) != hipSuccess ||
+                    (e = hipMalloc(&d.cell_list, size_t(2LL * d.radius * d.ncells * d.cell_cap) * sizeof(uint32_t))) !=
+                        hipSuccess) {
+                    return dev_fail(e, "hipMalloc");
+                }
+            }
         }
     }
     d.src_frame.assign(d.src_ring, -1);

@@ -2055,6 +2055,76 @@ constexpr int kTiledThreads = 512;
 constexpr int kTiledWaves = kTiledThreads / 64;
 constexpr int kTiledGroupsPerPass = kTiledWaves * 8;
 
+// One other-frame block of a group (entry e, the group's 8 lanes, lane c
+// holding column c of the filtered block in blk[0..7]) binned for
+// bm3d_gather: its weight once at its origin, its weighted values into a
+// record slot of the workgroup (rec_next counts the slots taken), the slot's
+// id into the lists of the up to 2 x 2 cells it covers (lane q < 4: cell q).
+// Where a list is full (or no slot is left) the block's pixels in that cell
+// are added atomically, as scatter_global does.
+template <int CM>
+__device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, const float *blk, float w, float wq,
+                                          int c, int lane, int bid, uint32_t *rec_next)
+{
+    const int z = int(entry_z(e));
+    const int k = z < A.radius ? z : z - 1;
+    const int bx = int(entry_x(e)), by = int(entry_y(e));
+    if (c == 0) {
+        unsafeAtomicAdd(A.origins + ((long long)k * A.height + by) * A.origin_pitch + bx, wq);
+    }
+    uint32_t slot = 0;
+    if (c == 0) {
+        slot = atomicAdd(rec_next, 1u);
+    }
+    slot = uint32_t(__shfl(int(slot), lane & ~7));
+    const bool slot_ok = slot < uint32_t(A.rec_per_tile);
+    const long long id = (long long)bid * A.rec_per_tile + slot;
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        v[j] = fma_c<CM>(w, blk[j], A.extractor) - A.extractor;
+    }
+    if (slot_ok) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            A.rec[id * 64 + j * 8 + c] = v[j];
+        }
+        if (c == 0) {
+            A.rec_pos[id] = e;
+        }
+    }
+    const int cx0 = bx >> kCellShift, cy0 = by >> kCellShift;
+    const int cx1 = (bx + 7) >> kCellShift, cy1 = (by + 7) >> kCellShift;
+    bool full = false;
+    {
+        const int qx = cx0 + (c & 1), qy = cy0 + (c >> 1);
+        if (c < 4 && qx <= cx1 && qy <= cy1) {
+            full = true;
+            if (slot_ok) {
+                const long long li = (long long)k * A.ncells + qy * A.cells_x + qx;
+                const uint32_t n = atomicAdd(A.cell_count + li, 1u);
+                if (n < uint32_t(A.cell_cap)) {
+                    A.cell_list[li * A.cell_cap + n] = uint32_t(id);
+                    full = false;
+                }
+            }
+        }
+    }
+    const uint32_t ovf = uint32_t(__ballot(full) >> (lane & ~7)) & 0xFu;
+    if (ovf != 0) {
+        const long long rp = A.res_pitch;
+        float *wd = A.res + (long long)z * 2 * A.height * rp + (long long)by * rp + bx + c;
+        const int qx = ((bx + c) >> kCellShift) - cx0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int qy = ((by + j) >> kCellShift) - cy0;
+            if ((ovf >> (qx + 2 * qy)) & 1u) {
+                unsafeAtomicAdd(wd + j * rp, v[j]);
+            }
+        }
+    }
+}
+
 // Tiled estimate: a workgroup filters the groups of a tile of tile_bx x
 // tile_by reference blocks.  Their blocks in the reference block's own frame
 // lie within bm_range of it, so those contributions are summed in an LDS
@@ -2063,12 +2133,14 @@ constexpr int kTiledGroupsPerPass = kTiledWaves * 8;
 // order-free -- then written out as floats (non-atomically) to the tile's
 // slot of the scratch buffer; the bm3d_tile_reduce pass adds the overlapping
 // tiles in a fixed order.  Blocks in the other frames of the temporal window
-// go to the dense slices with global atomics.
+// are binned by 64x64 cell of their slice (bin_block; bm3d_gather sums them),
+// or, without binning buffers, go to the dense slices with global atomics.
 template <int CM, bool FINAL, int MODE>
 __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArgs A)
 {
     extern __shared__ __align__(16) unsigned long long acc[];  // [2][region_h][region_w]
     __shared__ float tile[kTiledWaves][8][72];
+    __shared__ uint32_t rec_next;  // record slots taken (binned other-frame blocks)
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 3;
@@ -2085,6 +2157,9 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
     const int plane = A.region_h * rs;
     for (int i = threadIdx.x; i < 2 * plane; i += kTiledThreads) {
         acc[i] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        rec_next = 0u;
     }
     __syncthreads();
 
@@ -2128,6 +2203,8 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
                         __hip_atomic_fetch_add(a + plane + j * rs, wq_fixed, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
+                } else if (A.rec != nullptr && int(entry_z(ent[i])) != A.radius) {
+                    bin_block<CM>(A, ent[i], d + i * 8, w, wq, c, lane, bid, &rec_next);
                 } else {
                     scatter_global<CM>(A, ent[i], d + i * 8, w, wq, c);
                 }
@@ -2139,6 +2216,71 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
     const double scale = 1.0 / 4294967296.0;
     for (int i = threadIdx.x; i < 2 * plane; i += kTiledThreads) {
         out[i] = float(double((long long)acc[i]) * scale);
+    }
+}
+
+// The binned other-frame blocks of one 64x64 cell of one other slice
+// (blockIdx.x the cell, blockIdx.y the slice k, z = k or k + 1 past the
+// reference frame's), summed in LDS as 64-bit fixed point (exact and
+// independent of order, like the tile accumulators), then added to the
+// slice's value plane (the blocks added atomically where a list was full
+// are already there; nothing else writes these pixels meanwhile).  A
+// wavefront takes the list 64 records at a time (ids and positions a lane
+// each), then the records 8 at a time, a lane per pixel, all 8 loads issued
+// before the first add.
+constexpr int kCell = 1 << kCellShift;
+__global__ __launch_bounds__(256) void bm3d_gather(GatherArgs A)
+{
+    __shared__ unsigned long long acc[kCell * kCell];
+    const int cell = blockIdx.x, k = blockIdx.y;
+    const int cy = cell / A.cells_x, cx = cell - cy * A.cells_x;
+    const int x0 = cx << kCellShift, y0 = cy << kCellShift;
+    for (int i = threadIdx.x; i < kCell * kCell; i += 256) {
+        acc[i] = 0ull;
+    }
+    __syncthreads();
+    const long long li = (long long)k * A.ncells + cell;
+    const int n = __builtin_amdgcn_readfirstlane(min(int(A.cell_count[li]), A.cell_cap));
+    const uint32_t *list = A.cell_list + li * A.cell_cap;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    const int j = lane >> 3, c = lane & 7;
+    for (int b0 = wave * 64; b0 < n; b0 += 4 * 64) {
+        const int cnt = min(64, n - b0);
+        const uint32_t my_id = lane < cnt ? list[b0 + lane] : 0u;
+        const uint32_t my_e = lane < cnt ? A.rec_pos[my_id] : 0u;
+        for (int u0 = 0; u0 < cnt; u0 += 8) {
+            float v[8];
+            int at[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int src = min(u0 + u, cnt - 1);
+                const uint32_t id = uint32_t(__builtin_amdgcn_readlane(int(my_id), src));
+                const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(my_e), src));
+                const int px = int(entry_x(e)) + c - x0, py = int(entry_y(e)) + j - y0;
+                const bool ok = u0 + u < cnt && px >= 0 && px < kCell && py >= 0 && py < kCell;
+                at[u] = ok ? py * kCell + px : -1;
+                v[u] = ok ? A.rec[(long long)id * 64 + lane] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (at[u] >= 0) {
+                    __hip_atomic_fetch_add(&acc[at[u]], (unsigned long long)to_fixed(v[u]), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int z = k < A.radius ? k : k + 1;
+    float *slice = A.res + (long long)z * 2 * A.height * A.res_pitch;
+    const double scale = 1.0 / 4294967296.0;
+    for (int i = threadIdx.x; i < kCell * kCell; i += 256) {
+        const int x = x0 + (i & (kCell - 1)), y = y0 + (i >> kCellShift);
+        if (x < A.width && y < A.height && acc[i] != 0ull) {
+            float *p = slice + (long long)y * A.res_pitch + x;
+            *p = *p + float(double((long long)acc[i]) * scale);
+        }
     }
 }
 
@@ -2409,6 +2551,13 @@ hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s)
 {
     const dim3 grid((a.width + 63) / 64, (a.height + 3) / 4), block(256);
     hipLaunchKernelGGL(bm3d_tile_reduce, grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const GatherArgs &a, hipStream_t s)
+{
+    const dim3 grid(a.ncells, 2 * a.radius), block(256);
+    hipLaunchKernelGGL(bm3d_gather, grid, block, 0, s, a);
     return hipGetLastError();
 }
 
