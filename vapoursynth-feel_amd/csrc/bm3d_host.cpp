@@ -9,6 +9,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <cstdint>
 #include <condition_variable>
@@ -985,6 +986,10 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
                 d.cells_x = cx;
                 d.ncells = cx * cy;
                 d.cell_cap = 512;
+                if (const char *cap = std::getenv("VSFEEL_BM3D_CELL_CAP")) {
+                    // (tests: a small capacity sends most blocks down the atomic fallback)
+                    d.cell_cap = std::min(std::max(std::atoi(cap), 1), 4096);
+                }
                 d.cell_count_elems = (2LL * d.radius * d.ncells + 63) / 64 * 64;
                 if ((e = hipMalloc(&d.rec, size_t(tiles * per_tile) * 64 * sizeof(float))) != hipSuccess ||
                     (e = hipMalloc(&d.rec_pos, size_t(tiles * per_tile) * sizeof(uint32_t))) != hipSuccess ||
