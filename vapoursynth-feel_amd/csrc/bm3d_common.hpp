@@ -111,16 +111,28 @@ struct EstimateArgs {
     // radius > 0, tiled: the other frames' blocks are binned instead of
     // added with atomics.  A block's 64 weighted values go to a record slot
     // of its workgroup (rec, 64 floats per slot; rec_pos its entry); the
-    // record's id is appended to the list of every 64x64 cell of its slice
+    // record's id is appended to the list of every 32x32 cell of its slice
     // that the block covers (cell_list, cell_cap ids per cell, cell_count
     // appended so far).  bm3d_gather sums the lists into the slices.  A
     // block whose cell list is full adds its pixels in that cell atomically.
+    // A workgroup counts its appends per cell of its window (cwin_x x
+    // cwin_y cells per slice around its blocks, bin_halo pixels beyond
+    // them) in LDS and reserves each cell's share of the list at once.
     float *rec;                      // [tiles][rec_per_tile][64]
     uint32_t *rec_pos;               // [tiles][rec_per_tile]
     uint32_t *cell_count;            // [2r][ncells]
     uint32_t *cell_list;             // [2r][ncells][cell_cap]
     int rec_per_tile, cells_x, ncells, cell_cap;
+    int cwin_x, cwin_y, bin_halo;
 };
+
+// LDS bytes the binning adds to a tiled estimate workgroup (after its
+// accumulator): every record slot's entry, and per window cell its count,
+// list base and fill
+inline size_t bin_lds_bytes(int rec_per_tile, int radius, int cwin_x, int cwin_y)
+{
+    return sizeof(uint32_t) * (size_t(rec_per_tile) + 3 * size_t(2 * radius) * cwin_x * cwin_y);
+}
 
 constexpr int kCellShift = 5;        // binning cells of 32 x 32 pixels
 

@@ -153,11 +153,12 @@ struct Instance {
     float *origins = nullptr;    // radius > 0: 2r origin weight planes of height * pitch
     int *tile_tables = nullptr;  // col_tiles[W] (int2), row_tiles[H] (int2), tile_x0[tiles_x], tile_y0[tiles_y]
     bool reduce_span3 = false;   // every pixel lies in at most 3 x 3 tiles (bm3d_tile_reduce's unrolled form)
-    // radius > 0, tiled: the other frames' blocks binned by 64x64 cell (see
+    // radius > 0, tiled: the other frames' blocks binned by 32x32 cell (see
     // EstimateArgs::rec), summed by bm3d_gather
     float *rec = nullptr;
     uint32_t *rec_pos = nullptr, *cell_count = nullptr, *cell_list = nullptr;
     int rec_per_tile = 0, cells_x = 0, ncells = 0, cell_cap = 0;
+    int cwin_x = 0, cwin_y = 0, bin_halo = 0;
     long long cell_count_elems = 0;  // cleared per estimate (a multiple of 64)
     hipStream_t stream = nullptr;
     hipEvent_t last_done = nullptr;  // end of the previous device-side call
@@ -564,6 +565,9 @@ struct Instance {
             a.cells_x = cells_x;
             a.ncells = ncells;
             a.cell_cap = cell_cap;
+            a.cwin_x = cwin_x;
+            a.cwin_y = cwin_y;
+            a.bin_halo = bin_halo;
         }
         e = timed_launch(kKernelEstimate, s, [&] { return launch_estimate(a, fp_mode, final_pass, mode, s); });
         if (e == hipSuccess && rec && tiled) {
@@ -981,7 +985,16 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
             const long long rec_bytes = tiles * per_tile * 260;
             const int cw = 1 << kCellShift;
             const int cx = (d.width + cw - 1) / cw, cy = (d.height + cw - 1) / cw;
-            if (rec_bytes <= (4LL << 30) && tiles * per_tile < (1LL << 32)) {
+            // a workgroup's window: its blocks' pixels and the temporal
+            // search's reach beyond them (spatial range plus ps_range per step)
+            const int halo = d.bm_range + d.radius * d.ps_range;
+            const int wx = ((d.tile_bx - 1) * d.block_step + 2 * halo + 7) / cw + 2;
+            const int wy = ((d.tile_by - 1) * d.block_step + 2 * halo + 7) / cw + 2;
+            const size_t lds = size_t(2) * d.region_h * d.region_w * 8 + bin_lds_bytes(int(per_tile), d.radius, wx, wy);
+            if (rec_bytes <= (4LL << 30) && tiles * per_tile < (1LL << 32) && lds + 20 * 1024 <= 160 * 1024) {
+                d.cwin_x = wx;
+                d.cwin_y = wy;
+                d.bin_halo = halo;
                 d.rec_per_tile = int(per_tile);
                 d.cells_x = cx;
                 d.ncells = cx * cy;

@@ -2058,13 +2058,17 @@ constexpr int kTiledGroupsPerPass = kTiledWaves * 8;
 // One other-frame block of a group (entry e, the group's 8 lanes, lane c
 // holding column c of the filtered block in blk[0..7]) binned for
 // bm3d_gather: its weight once at its origin, its weighted values into a
-// record slot of the workgroup (rec_next counts the slots taken), the slot's
-// id into the lists of the up to 2 x 2 cells it covers (lane q < 4: cell q).
-// Where a list is full (or no slot is left) the block's pixels in that cell
-// are added atomically, as scatter_global does.
+// record slot of the workgroup (rec_next counts the slots taken, rec_e
+// keeps their entries), and for each of the up to 2 x 2 cells it covers
+// (lane q < 4: cell q) a count in the workgroup's window (lcount; the slot
+// goes to the cell's list after the last group) or, for a cell outside the
+// window, the slot's id appended to the cell's list at once.  Where that
+// list is full (or no slot is left) the block's pixels in the cell are
+// added atomically, as scatter_global does.
 template <int CM>
 __device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, const float *blk, float w, float wq,
-                                          int c, int lane, int bid, uint32_t *rec_next)
+                                          int c, int lane, int bid, uint32_t *rec_next, uint32_t *rec_e,
+                                          uint32_t *lcount, int cwx0, int cwy0)
 {
     const int z = int(entry_z(e));
     const int k = z < A.radius ? z : z - 1;
@@ -2091,6 +2095,7 @@ __device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, con
         }
         if (c == 0) {
             A.rec_pos[id] = e;
+            rec_e[slot] = e;
         }
     }
     const int cx0 = bx >> kCellShift, cy0 = by >> kCellShift;
@@ -2101,11 +2106,17 @@ __device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, con
         if (c < 4 && qx <= cx1 && qy <= cy1) {
             full = true;
             if (slot_ok) {
-                const long long li = (long long)k * A.ncells + qy * A.cells_x + qx;
-                const uint32_t n = atomicAdd(A.cell_count + li, 1u);
-                if (n < uint32_t(A.cell_cap)) {
-                    A.cell_list[li * A.cell_cap + n] = uint32_t(id);
+                const int lx = qx - cwx0, ly = qy - cwy0;
+                if (lx >= 0 && lx < A.cwin_x && ly >= 0 && ly < A.cwin_y) {
+                    atomicAdd(lcount + (k * A.cwin_y + ly) * A.cwin_x + lx, 1u);
                     full = false;
+                } else {
+                    const long long li = (long long)k * A.ncells + qy * A.cells_x + qx;
+                    const uint32_t n = atomicAdd(A.cell_count + li, 1u);
+                    if (n < uint32_t(A.cell_cap)) {
+                        A.cell_list[li * A.cell_cap + n] = uint32_t(id);
+                        full = false;
+                    }
                 }
             }
         }
@@ -2133,7 +2144,7 @@ __device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, con
 // order-free -- then written out as floats (non-atomically) to the tile's
 // slot of the scratch buffer; the bm3d_tile_reduce pass adds the overlapping
 // tiles in a fixed order.  Blocks in the other frames of the temporal window
-// are binned by 64x64 cell of their slice (bin_block; bm3d_gather sums them),
+// are binned by 32x32 cell of their slice (bin_block; bm3d_gather sums them),
 // or, without binning buffers, go to the dense slices with global atomics.
 template <int CM, bool FINAL, int MODE>
 __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArgs A)
@@ -2141,6 +2152,7 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
     extern __shared__ __align__(16) unsigned long long acc[];  // [2][region_h][region_w]
     __shared__ float tile[kTiledWaves][8][72];
     __shared__ uint32_t rec_next;  // record slots taken (binned other-frame blocks)
+    __shared__ int cwin0[2];       // the binning window's first cell (x, y)
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 3;
@@ -2155,15 +2167,29 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
     tile_span(ty, A.tile_by, A.ny, A.step, A.height, A.halo, &y_lo, &y_hi);
     const int rs = A.region_w;
     const int plane = A.region_h * rs;
+    // binning (radius > 0): after the accumulator, every record slot's
+    // entry, then per window cell its count, list base and fill
+    uint32_t *rec_e = reinterpret_cast<uint32_t *>(acc + 2 * plane);
+    const int nloc = A.rec != nullptr ? 2 * A.radius * A.cwin_x * A.cwin_y : 0;
+    uint32_t *lcount = rec_e + A.rec_per_tile;
+    uint32_t *lbase = lcount + nloc;
+    uint32_t *lfill = lbase + nloc;
     for (int i = threadIdx.x; i < 2 * plane; i += kTiledThreads) {
         acc[i] = 0ull;
     }
+    for (int i = threadIdx.x; i < nloc; i += kTiledThreads) {
+        lcount[i] = 0u;
+        lfill[i] = 0u;
+    }
+    const int bx0 = tx * A.tile_bx, by0 = ty * A.tile_by;
     if (threadIdx.x == 0) {
         rec_next = 0u;
+        cwin0[0] = max(min(bx0 * A.step, A.width - 8) - A.bin_halo, 0) >> kCellShift;
+        cwin0[1] = max(min(by0 * A.step, A.height - 8) - A.bin_halo, 0) >> kCellShift;
     }
     __syncthreads();
+    const int cwx0 = cwin0[0], cwy0 = cwin0[1];
 
-    const int bx0 = tx * A.tile_bx, by0 = ty * A.tile_by;
     const int nbx = min(A.tile_bx, A.nx - bx0);
     const int nby = min(A.tile_by, A.ny - by0);
     const int nt = nbx * nby;
@@ -2204,7 +2230,7 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                 } else if (A.rec != nullptr && int(entry_z(ent[i])) != A.radius) {
-                    bin_block<CM>(A, ent[i], d + i * 8, w, wq, c, lane, bid, &rec_next);
+                    bin_block<CM>(A, ent[i], d + i * 8, w, wq, c, lane, bid, &rec_next, rec_e, lcount, cwx0, cwy0);
                 } else {
                     scatter_global<CM>(A, ent[i], d + i * 8, w, wq, c);
                 }
@@ -2212,6 +2238,53 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
         }
     }
     __syncthreads();
+    if (nloc > 0) {
+        // each window cell's share of its list, then the records into it
+        for (int i = threadIdx.x; i < nloc; i += kTiledThreads) {
+            const uint32_t n = lcount[i];
+            if (n != 0u) {
+                const int lx = i % A.cwin_x, t2 = i / A.cwin_x;
+                const int ly = t2 % A.cwin_y, k = t2 / A.cwin_y;
+                const long long li = (long long)k * A.ncells + (cwy0 + ly) * A.cells_x + cwx0 + lx;
+                lbase[i] = atomicAdd(A.cell_count + li, n);
+            }
+        }
+        __syncthreads();
+        const int nrec = min(int(rec_next), A.rec_per_tile);
+        const long long rp = A.res_pitch;
+        for (int sl = threadIdx.x; sl < nrec; sl += kTiledThreads) {
+            const uint32_t e = rec_e[sl];
+            const uint32_t id = uint32_t(bid) * uint32_t(A.rec_per_tile) + uint32_t(sl);
+            const int z = int(entry_z(e));
+            const int k = z < A.radius ? z : z - 1;
+            const int bx = int(entry_x(e)), by = int(entry_y(e));
+            const int cx0 = bx >> kCellShift, cy0 = by >> kCellShift;
+            const int cx1 = (bx + 7) >> kCellShift, cy1 = (by + 7) >> kCellShift;
+            for (int qy = cy0; qy <= cy1; ++qy) {
+                for (int qx = cx0; qx <= cx1; ++qx) {
+                    const int lx = qx - cwx0, ly = qy - cwy0;
+                    if (lx < 0 || lx >= A.cwin_x || ly < 0 || ly >= A.cwin_y) {
+                        continue;  // appended by bin_block
+                    }
+                    const int lc = (k * A.cwin_y + ly) * A.cwin_x + lx;
+                    const uint32_t n = lbase[lc] + atomicAdd(lfill + lc, 1u);
+                    const long long li = (long long)k * A.ncells + qy * A.cells_x + qx;
+                    if (n < uint32_t(A.cell_cap)) {
+                        A.cell_list[li * A.cell_cap + n] = id;
+                    } else {
+                        // the list is full: the record's pixels in this cell atomically
+                        float *wd = A.res + (long long)z * 2 * A.height * rp;
+                        for (int p = 0; p < 64; ++p) {
+                            const int px = bx + (p & 7), py = by + (p >> 3);
+                            if ((px >> kCellShift) == qx && (py >> kCellShift) == qy) {
+                                unsafeAtomicAdd(wd + (long long)py * rp + px, A.rec[(long long)id * 64 + p]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
     float *out = A.scratch + (long long)bid * 2 * plane;
     const double scale = 1.0 / 4294967296.0;
     for (int i = threadIdx.x; i < 2 * plane; i += kTiledThreads) {
@@ -2219,7 +2292,7 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
     }
 }
 
-// The binned other-frame blocks of one 64x64 cell of one other slice
+// The binned other-frame blocks of one 32x32 cell of one other slice
 // (blockIdx.x the cell, blockIdx.y the slice k, z = k or k + 1 past the
 // reference frame's), summed in LDS as 64-bit fixed point (exact and
 // independent of order, like the tile accumulators), then added to the
@@ -2510,7 +2583,9 @@ hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, 
 {
     const bool tiled = a.scratch != nullptr;
     const dim3 grid(tiled ? a.tiles_x * a.tiles_y : (a.ngroups + 31) / 32), block(tiled ? kTiledThreads : 256);
-    const size_t lds = tiled ? size_t(2) * a.region_h * a.region_w * sizeof(unsigned long long) : 0;
+    const size_t lds = tiled ? size_t(2) * a.region_h * a.region_w * sizeof(unsigned long long) +
+                                   (a.rec != nullptr ? bin_lds_bytes(a.rec_per_tile, a.radius, a.cwin_x, a.cwin_y) : 0)
+                             : 0;
 #define VSFEEL_EST(CM, FIN, MODE)                                                              \
     if (tiled) {                                                                               \
         hipLaunchKernelGGL((bm3d_estimate_tiled<CM, FIN, MODE>), grid, block, lds, s, a);      \
