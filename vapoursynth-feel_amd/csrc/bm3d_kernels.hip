@@ -2299,8 +2299,8 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
 // slice's value plane (the blocks added atomically where a list was full
 // are already there; nothing else writes these pixels meanwhile).  A
 // wavefront takes the list 64 records at a time (ids and positions a lane
-// each), then the records 8 at a time, a lane per pixel, all 8 loads issued
-// before the first add.
+// each), then the records 16 at a time, a lane per pixel, all 16 loads
+// issued before the first add.
 constexpr int kCell = 1 << kCellShift;
 __global__ __launch_bounds__(256) void bm3d_gather(GatherArgs A)
 {
@@ -2322,11 +2322,11 @@ __global__ __launch_bounds__(256) void bm3d_gather(GatherArgs A)
         const int cnt = min(64, n - b0);
         const uint32_t my_id = lane < cnt ? list[b0 + lane] : 0u;
         const uint32_t my_e = lane < cnt ? A.rec_pos[my_id] : 0u;
-        for (int u0 = 0; u0 < cnt; u0 += 8) {
-            float v[8];
-            int at[8];
+        for (int u0 = 0; u0 < cnt; u0 += 16) {
+            float v[16];
+            int at[16];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 16; ++u) {
                 const int src = min(u0 + u, cnt - 1);
                 const uint32_t id = uint32_t(__builtin_amdgcn_readlane(int(my_id), src));
                 const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(my_e), src));
@@ -2336,7 +2336,7 @@ __global__ __launch_bounds__(256) void bm3d_gather(GatherArgs A)
                 v[u] = ok ? A.rec[(long long)id * 64 + lane] : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 16; ++u) {
                 if (at[u] >= 0) {
                     __hip_atomic_fetch_add(&acc[at[u]], (unsigned long long)to_fixed(v[u]), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
