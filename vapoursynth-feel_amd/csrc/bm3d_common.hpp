@@ -110,18 +110,17 @@ struct EstimateArgs {
     int origin_pitch;
     // radius > 0, tiled: the other frames' blocks are binned instead of
     // added with atomics.  A block's 64 weighted values go to a record slot
-    // of its workgroup (rec, 64 floats per slot; rec_pos its entry); the
+    // of its workgroup (rec, 64 floats per slot); the
     // record's id is appended to the list of every 32x32 cell of its slice
-    // that the block covers (cell_list, cell_cap ids per cell, cell_count
+    // that the block covers (cell_list, cell_cap (id, entry) pairs per cell, cell_count
     // appended so far).  bm3d_gather sums the lists into the slices.  A
     // block whose cell list is full adds its pixels in that cell atomically.
     // A workgroup counts its appends per cell of its window (cwin_x x
     // cwin_y cells per slice around its blocks, bin_halo pixels beyond
     // them) in LDS and reserves each cell's share of the list at once.
     float *rec;                      // [tiles][rec_per_tile][64]
-    uint32_t *rec_pos;               // [tiles][rec_per_tile]
     uint32_t *cell_count;            // [2r][ncells]
-    uint32_t *cell_list;             // [2r][ncells][cell_cap]
+    uint2 *cell_list;                // [2r][ncells][cell_cap] (record id, entry)
     int rec_per_tile, cells_x, ncells, cell_cap;
     int cwin_x, cwin_y, bin_halo;
 };
@@ -140,9 +139,8 @@ constexpr int kCellShift = 5;        // binning cells of 32 x 32 pixels
 // estimate stack's other slices (one workgroup per cell and slice).
 struct GatherArgs {
     const float *rec;
-    const uint32_t *rec_pos;
     const uint32_t *cell_count;
-    const uint32_t *cell_list;
+    const uint2 *cell_list;
     float *res;                      // the estimate stack, [2r+1][2][height][res_pitch]
     int width, height, res_pitch;
     int cells_x, ncells, cell_cap, radius;

@@ -156,7 +156,8 @@ struct Instance {
     // radius > 0, tiled: the other frames' blocks binned by 32x32 cell (see
     // EstimateArgs::rec), summed by bm3d_gather
     float *rec = nullptr;
-    uint32_t *rec_pos = nullptr, *cell_count = nullptr, *cell_list = nullptr;
+    uint32_t *cell_count = nullptr;
+    uint2 *cell_list = nullptr;
     int rec_per_tile = 0, cells_x = 0, ncells = 0, cell_cap = 0;
     int cwin_x = 0, cwin_y = 0, bin_halo = 0;
     long long cell_count_elems = 0;  // cleared per estimate (a multiple of 64)
@@ -395,7 +396,6 @@ struct Instance {
         (void)hipFree(origins);
         (void)hipFree(tile_tables);
         (void)hipFree(rec);
-        (void)hipFree(rec_pos);
         (void)hipFree(cell_count);
         (void)hipFree(cell_list);
         if (last_done) {
@@ -558,7 +558,6 @@ struct Instance {
             a.region_w = region_w;
             a.region_h = region_h;
             a.rec = rec;
-            a.rec_pos = rec_pos;
             a.cell_count = cell_count;
             a.cell_list = cell_list;
             a.rec_per_tile = rec_per_tile;
@@ -573,7 +572,6 @@ struct Instance {
         if (e == hipSuccess && rec && tiled) {
             GatherArgs ga {};
             ga.rec = rec;
-            ga.rec_pos = rec_pos;
             ga.cell_count = cell_count;
             ga.cell_list = cell_list;
             ga.res = stack;
@@ -982,7 +980,7 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
             // take more than 4 GiB
             const long long tiles = (long long)d.tiles_x * d.tiles_y;
             const long long per_tile = (long long)d.tile_bx * d.tile_by * 7;
-            const long long rec_bytes = tiles * per_tile * 260;
+            const long long rec_bytes = tiles * per_tile * 256;
             const int cw = 1 << kCellShift;
             const int cx = (d.width + cw - 1) / cw, cy = (d.height + cw - 1) / cw;
             // a workgroup's window: its blocks' pixels and the temporal
@@ -1005,9 +1003,8 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
                 }
                 d.cell_count_elems = (2LL * d.radius * d.ncells + 63) / 64 * 64;
                 if ((e = hipMalloc(&d.rec, size_t(tiles * per_tile) * 64 * sizeof(float))) != hipSuccess ||
-                    (e = hipMalloc(&d.rec_pos, size_t(tiles * per_tile) * sizeof(uint32_t))) != hipSuccess ||
                     (e = hipMalloc(&d.cell_count, size_t(d.cell_count_elems) * sizeof(uint32_t))) != hipSuccess ||
-                    (e = hipMalloc(&d.cell_list, size_t(2LL * d.radius * d.ncells * d.cell_cap) * sizeof(uint32_t))) !=
+                    (e = hipMalloc(&d.cell_list, size_t(2LL * d.radius * d.ncells * d.cell_cap) * sizeof(uint2))) !=
                         hipSuccess) {
                     return dev_fail(e, "hipMalloc");
                 }

@@ -2094,7 +2094,6 @@ __device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, con
             A.rec[id * 64 + j * 8 + c] = v[j];
         }
         if (c == 0) {
-            A.rec_pos[id] = e;
             rec_e[slot] = e;
         }
     }
@@ -2114,7 +2113,7 @@ __device__ __forceinline__ void bin_block(const EstimateArgs &A, uint32_t e, con
                     const long long li = (long long)k * A.ncells + qy * A.cells_x + qx;
                     const uint32_t n = atomicAdd(A.cell_count + li, 1u);
                     if (n < uint32_t(A.cell_cap)) {
-                        A.cell_list[li * A.cell_cap + n] = uint32_t(id);
+                        A.cell_list[li * A.cell_cap + n] = make_uint2(uint32_t(id), e);
                         full = false;
                     }
                 }
@@ -2270,7 +2269,7 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
                     const uint32_t n = lbase[lc] + atomicAdd(lfill + lc, 1u);
                     const long long li = (long long)k * A.ncells + qy * A.cells_x + qx;
                     if (n < uint32_t(A.cell_cap)) {
-                        A.cell_list[li * A.cell_cap + n] = id;
+                        A.cell_list[li * A.cell_cap + n] = make_uint2(id, e);
                     } else {
                         // the list is full: the record's pixels in this cell atomically
                         float *wd = A.res + (long long)z * 2 * A.height * rp;
@@ -2298,7 +2297,7 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
 // independent of order, like the tile accumulators), then added to the
 // slice's value plane (the blocks added atomically where a list was full
 // are already there; nothing else writes these pixels meanwhile).  A
-// wavefront takes the list 64 records at a time (ids and positions a lane
+// wavefront takes the list 64 records at a time (ids and entry words a lane
 // each), then the records 16 at a time, a lane per pixel, all 16 loads
 // issued before the first add.
 constexpr int kCell = 1 << kCellShift;
@@ -2314,14 +2313,14 @@ __global__ __launch_bounds__(256) void bm3d_gather(GatherArgs A)
     __syncthreads();
     const long long li = (long long)k * A.ncells + cell;
     const int n = __builtin_amdgcn_readfirstlane(min(int(A.cell_count[li]), A.cell_cap));
-    const uint32_t *list = A.cell_list + li * A.cell_cap;
+    const uint2 *list = A.cell_list + li * A.cell_cap;
     const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     const int j = lane >> 3, c = lane & 7;
     for (int b0 = wave * 64; b0 < n; b0 += 4 * 64) {
         const int cnt = min(64, n - b0);
-        const uint32_t my_id = lane < cnt ? list[b0 + lane] : 0u;
-        const uint32_t my_e = lane < cnt ? A.rec_pos[my_id] : 0u;
+        const uint2 my = lane < cnt ? list[b0 + lane] : make_uint2(0u, 0u);
+        const uint32_t my_id = my.x, my_e = my.y;
         for (int u0 = 0; u0 < cnt; u0 += 16) {
             float v[16];
             int at[16];
