@@ -134,12 +134,13 @@ def test_groups_and_output_match_oracle(name):
     assert err <= TOL, f"{name}: max |gpu - oracle| = {err}"
 
 
-@pytest.mark.parametrize("name", ["c3_small", "r1", "stripes"])
+@pytest.mark.parametrize("name", ["c3_small", "four_predictors", "wide_predictive", "r4_exact"])
 def test_binning_overflow_matches_oracle(name, monkeypatch):
-    """At radius > 0 the other frames' blocks are binned by 32x32 cell; a
-    block whose cell list is full adds its pixels there atomically.  A list
-    capacity of 3 sends most blocks down that path; the frames must still
-    match the oracle (bit-exact with the extractor)."""
+    """Where a group takes at least 8 temporal insertions (2 radius ps_num)
+    the other frames' blocks are binned by 32x32 cell; a block whose cell
+    list is full adds its pixels there atomically.  A list capacity of 3
+    sends most blocks down that path; the frames must still match the
+    oracle (bit-exact with the extractor)."""
     monkeypatch.setenv("VSFEEL_BM3D_CELL_CAP", "3")
     got, ref, kw = _run_case(name)
     err = float(np.abs(got - ref).max())

@@ -974,10 +974,12 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
             hipSuccess) {
             return dev_fail(e, "hipMemcpy");
         }
-        if (d.radius > 0) {
-            // a group holds at most 7 other-frame blocks (the self block is
-            // always a member); binning is skipped where its records would
-            // take more than 4 GiB
+        if (d.radius > 0 && 2 * d.radius * d.ps_num >= 8) {
+            // binning pays where a group takes at least 8 temporal
+            // insertions (at 4K, radius 1, ps_num 2 the gather cost more
+            // than the atomics it replaced); a group holds at most 7
+            // other-frame blocks (the self block is always a member);
+            // binning is skipped where its records would take more than 4 GiB
             const long long tiles = (long long)d.tiles_x * d.tiles_y;
             const long long per_tile = (long long)d.tile_bx * d.tile_by * 7;
             const long long rec_bytes = tiles * per_tile * 256;
