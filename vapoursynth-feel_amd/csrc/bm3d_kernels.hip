@@ -493,6 +493,57 @@ __device__ __forceinline__ int half_prefix_sum32(int v, int lane)
     return (lane & 16) ? v + below : v;
 }
 
+// minimum over the whole wavefront in every lane: the half minimum, then
+// v_permlane32_swap hands each lane the other half's
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v, int lane)
+{
+    v = half_min32_u32(v, lane);
+    const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return min(uint32_t(sw[0]), uint32_t(sw[1]));
+}
+
+// Merges one candidate per lane (kNoKey where there is none) into the list
+// (lanes 0..7), keeping its k smallest keys in lanes 0..k-1 (k <= 8; lanes
+// k..7 then hold placeholders): k rounds of a wavefront minimum over each
+// lane's smaller remaining key, the winning lane (the lowest among equal
+// keys) moving on to its other one.  No LDS; for the few keys a tie-heavy
+// search needs, cheaper than sorting all 64.
+__device__ void list_merge_rounds(TopList &L, uint64_t key, uint32_t xy, int lane, int k)
+{
+    uint64_t lo = key, hi = lane < 8 ? L.key : kNoKey;
+    uint32_t lxy = xy, hxy = lane < 8 ? L.xy : 0u;
+    if (hi < lo) {
+        const uint64_t t = lo;
+        lo = hi;
+        hi = t;
+        const uint32_t tx = lxy;
+        lxy = hxy;
+        hxy = tx;
+    }
+    uint64_t nk = kPlaceholderKey;
+    uint32_t nxy = 0u;
+    for (int r = 0; r < k; ++r) {
+        const uint32_t mh = wave_min_u32(uint32_t(lo >> 32), lane);
+        const uint32_t ml = wave_min_u32(uint32_t(lo >> 32) == mh ? uint32_t(lo) : ~0u, lane);
+        const uint64_t win = __ballot(uint32_t(lo >> 32) == mh && uint32_t(lo) == ml);
+        const int wl = __ffsll((unsigned long long)win) - 1;
+        const uint32_t wxy = uint32_t(__builtin_amdgcn_readlane(int(lxy), wl));
+        if (lane == r) {
+            nk = (uint64_t(mh) << 32) | ml;
+            nxy = wxy;
+        }
+        if (lane == wl) {
+            lo = hi;
+            lxy = hxy;
+            hi = kNoKey;
+        }
+    }
+    if (lane < 8) {
+        L.key = nk < kPlaceholderKey ? nk : kPlaceholderKey;
+        L.xy = nk < kPlaceholderKey ? nxy : 0u;
+    }
+}
+
 template <int K, int J>
 __device__ __forceinline__ void sort16_step(uint64_t &key, uint32_t &xy, int lane)
 {
@@ -908,7 +959,7 @@ __device__ __forceinline__ void merge_staged(const Geo &geo, uint32_t thr, int s
             key = make_key(d, geo.seq(r, k));
         }
     }
-    list_merge_batch(L, key, kxy, lane);
+    list_merge_rounds(L, key, kxy, lane, 8);
 }
 
 // The same search by a whole wavefront, for more than 16 staged candidates
@@ -1267,29 +1318,30 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 // which gives the group the shader's sequence of insertions.
 // ---------------------------------------------------------------------------
 
-// the frame top-8 of half h when more than 16 candidates survive (ties): its
-// rows holding one are scanned in seq order by the whole wavefront, the
-// survivors merged into a list in batches of 64; once the list holds 8, only
-// distances below its 8th can still enter (equal ones lose on seq), so the
-// rest of a window of equal distances is skipped; returns it in lanes 0..7
+// the frame's first `need` (distance, seq) keys of half h (all a step uses)
+// when more than 16 candidates survive (ties): its rows holding one are
+// scanned in seq order by the whole wavefront, the survivors merged into a
+// list in batches of 64; once the list holds `need`, only distances below
+// its last can still enter (equal ones lose on seq), so the rest of a window
+// of equal distances is skipped; returns them in lanes 0..need-1
 __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint32_t thr_h, uint32_t pm,
                                        const float *tdist, int wh, int w, int rr, int rw, int left, int top,
-                                       unsigned long long *sk, uint32_t *sxy, int lane)
+                                       unsigned long long *sk, uint32_t *sxy, int lane, int need)
 {
     const int win = wh * wh;
     TopList L;
     list_reset(L);
     int staged = 0;
     uint32_t rows = uint32_t(__ballot((lane >> 5) == h && row_ok && (m.x & ~pm) <= thr_h) >> (h * 32));
-    uint32_t lim = kFltMaxBits;  // the list's 8th distance (placeholders: FLT_MAX)
+    uint32_t lim = kFltMaxBits;  // the list's last distance (placeholders: FLT_MAX)
     auto merge = [&]() {
         wave_lds_sync();
         const uint64_t key = lane < staged ? sk[lane] : kNoKey;
         const uint32_t kxy = lane < staged ? sxy[lane] : 0u;
         wave_lds_sync();
-        list_merge_batch(L, key, kxy, lane);
+        list_merge_rounds(L, key, kxy, lane, need);
         staged = 0;
-        lim = uint32_t(list_threshold(L) >> 32);
+        lim = uint32_t(readlane_u64(L.key, need - 1) >> 32);
     };
     while (rows) {
         const int src = h * 32 + __ffs(rows) - 1;
@@ -1534,7 +1586,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         for (int h = 0; h < 2; ++h) {
             if (__builtin_amdgcn_readlane(staged, h * 32) > 16) {
                 const TopList F = predictive_fallback(h, m, row_ok, uint32_t(__builtin_amdgcn_readlane(int(thr), h * 32)),
-                                                      pm, tdist, wh, w, rr, rw, left, top, sk, sxy, lane);
+                                                      pm, tdist, wh, w, rr, rw, left, top, sk, sxy, lane, PN);
                 const uint64_t fk = (uint64_t(uint32_t(__shfl(int(uint32_t(F.key >> 32)), sl & 7))) << 32) |
                                     uint32_t(__shfl(int(uint32_t(F.key)), sl & 7));
                 const uint32_t fx = uint32_t(__shfl(int(F.xy), sl & 7));
