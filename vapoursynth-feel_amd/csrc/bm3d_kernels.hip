@@ -1342,6 +1342,8 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
         list_merge_rounds(L, key, kxy, lane, need);
         staged = 0;
         lim = uint32_t(readlane_u64(L.key, need - 1) >> 32);
+        // rows whose smallest key cannot beat the list's last are done
+        rows &= uint32_t(__ballot((lane >> 5) == h && (m.x & ~pm) < lim) >> (h * 32));
     };
     while (rows) {
         const int src = h * 32 + __ffs(rows) - 1;
@@ -1369,7 +1371,7 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
             sxy[p] = uint32_t(sl + lane) | (uint32_t(st + sr) << 16);
         }
         staged += cnt;
-        if (staged >= 16 && lim == kFltMaxBits) {
+        if (staged >= need && lim == kFltMaxBits) {
             merge();  // enough for a full list: merge now to start skipping
         }
     }
