@@ -1442,27 +1442,31 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
             const int p = p0 + half;
             const bool on = p < PN && sl < wc;
             float v[2][kPredBatch];
-            uint32_t base[2];  // byte offsets in the frame (a plane is below 4 GiB)
-            int nrow[2];
+            // byte offsets in the frame (a plane is below 4 GiB): the lane's
+            // column in the window's first row and in its last row; a row's
+            // offset steps by pitch4 and is clamped to the last row's
+            uint32_t base[2], lastrow[2];
+            const uint32_t pitch4 = uint32_t(pitch) * 4u;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const uint32_t q = uint32_t(__shfl(int(pred), h * 32 + min(p, PN - 1)));
                 const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
                 const int left = max(cx - PR, 0), top = max(cy - PR, 0);
                 const int ncol = min(cx + PR, W - 8) - left + 8;
-                nrow[h] = min(cy + PR, H - 8) - top + 8;
+                const int nrow = min(cy + PR, H - 8) - top + 8;
                 base[h] = uint32_t(top * pitch + left + min(sl, ncol - 1)) * 4u;
+                lastrow[h] = base[h] + uint32_t(nrow - 1) * pitch4;
             }
-            const uint32_t pitch4 = uint32_t(pitch) * 4u;
             for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
                 if (on) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         const char *plane = reinterpret_cast<const char *>(A.plane[h ? r + t : r - t]);
+                        uint32_t off = base[h] + uint32_t(i0) * pitch4;
 #pragma unroll
                         for (int u = 0; u < kPredBatch; ++u) {
-                            const uint32_t off = base[h] + uint32_t(min(i0 + u, nrow[h] - 1)) * pitch4;
-                            v[h][u] = *reinterpret_cast<const float *>(plane + off);
+                            v[h][u] = *reinterpret_cast<const float *>(plane + min(off, lastrow[h]));
+                            off += pitch4;
                         }
                     }
 #pragma unroll
