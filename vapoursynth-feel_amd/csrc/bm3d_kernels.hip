@@ -1618,8 +1618,8 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         const int e_idx = c0 + (lane - 8);
         const bool entry = lane >= 8 && lane < 16 && e_idx < n_ins;
         const int ei = entry ? e_idx : 0;
-        const int ht = ei / PN, i = ei - ht * PN;
-        const int h = ht / r, t = ht - h * r + 1;
+        const int ht = div_small(ei, 1.0f / float(PN)), i = ei - ht * PN;
+        const int h = div_small(ht, 1.0f / float(r)), t = ht - h * r + 1;
         const int src = h * 32 + i;
         const uint32_t e1 = uint32_t(__shfl(int(uint32_t(fkey[0] >> 32)), src));
         const uint32_t e2 = uint32_t(__shfl(int(uint32_t(fkey[1] >> 32)), src));
