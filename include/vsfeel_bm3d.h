@@ -221,7 +221,8 @@ int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);
 /*
  * Graph replay of vsfeel_bm3d_process_device (on by default): a call
  * captures its launch sequence (per output frame: slot clears, matching,
- * filtering, tile reduction, aggregation) into a HIP graph on the instance's
+ * filtering, the other frames' sums and weights at radius > 0, tile
+ * reduction, aggregation) into a HIP graph on the instance's
  * stream, and later calls with identical arguments replay it, ordered after
  * and before the caller's stream by events.  The counterpart of the
  * reference's per-frame command-buffer recording (src/bm3d.cpp:486-623).
