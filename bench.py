@@ -248,8 +248,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the path shards by frame across GPUs "
-              "(see DESIGN.md) but no multi-GPU run is part of it", file=sys.stderr)
+        print("bench.py: measures one GPU only; frames would shard across GPUs (each GPU "
+              "filtering its own frames, the temporal window's neighbours read from its "
+              "peers), but no multi-GPU run is implemented", file=sys.stderr)
         sys.exit(2)
 
     forms = [f for f in args.forms.split(",") if f]
