@@ -249,8 +249,8 @@ def main():
 
     if args.gpus != 1:
         print("bench.py: measures one GPU only; frames would shard across GPUs (each GPU "
-              "filtering its own frames, the temporal window's neighbours read from its "
-              "peers), but no multi-GPU run is implemented", file=sys.stderr)
+              "filtering a contiguous range of frames from its own copy of the range and a "
+              "2r-frame source halo), but no multi-GPU run is implemented", file=sys.stderr)
         sys.exit(2)
 
     forms = [f for f in args.forms.split(",") if f]
