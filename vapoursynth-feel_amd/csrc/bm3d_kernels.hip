@@ -2255,7 +2255,8 @@ __global__ __launch_bounds__(kTiledThreads) void bm3d_estimate_tiled(EstimateArg
         const int k = k0 + wave * 8 + g;
         const bool active = k < nt;
         const int kk = active ? k : 0;
-        const int gid = (by0 + kk / nbx) * A.nx + bx0 + kk % nbx;
+        const int kq = div_small(kk, 1.0f / float(nbx));  // kk < 2^16
+        const int gid = (by0 + kq) * A.nx + bx0 + (kk - kq * nbx);
         uint32_t ent[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
