@@ -32,6 +32,12 @@ CASES = {
     # the 1080p parameters (step 4, range 16) on a smaller frame
     "c2_small": (lambda s: s.workload(2, frames=2, width=480, height=272),
                  dict(sigma=3.0, block_step=4, bm_range=16, radius=0)),
+    # the benchmark sizes: 1080p plain (frame 0, and frame 7 whose texture
+    # runs along the rows), and 1080p radius 2 over 5 frames
+    "c2_full": (lambda s: s.workload(2, frames=8)[[0, 7]],
+                dict(sigma=3.0, block_step=4, bm_range=16, radius=0)),
+    "c3_full": (lambda s: s.workload(3, frames=5)[0],
+                dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7, extractor_exp=2)),
     # temporal: radius 2, predictive search 2/7, extractor 2^2, moving content
     "c3_small": (lambda s: s.workload(3, frames=6, width=256, height=144)[0],
                  dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7,
