@@ -38,6 +38,9 @@ CASES = {
                 dict(sigma=3.0, block_step=4, bm_range=16, radius=0)),
     "c3_full": (lambda s: s.workload(3, frames=5)[0],
                 dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7, extractor_exp=2)),
+    # the precision-form workload's shape: 3840x2160, radius 1 (3 frames)
+    "c4_full": (lambda s: s.workload(4, frames=3),
+                dict(sigma=5.0, block_step=4, bm_range=16, radius=1, ps_num=2, ps_range=7)),
     # temporal: radius 2, predictive search 2/7, extractor 2^2, moving content
     "c3_small": (lambda s: s.workload(3, frames=6, width=256, height=144)[0],
                  dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7,
