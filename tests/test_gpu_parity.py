@@ -38,6 +38,10 @@ CASES = {
                 dict(sigma=3.0, block_step=4, bm_range=16, radius=0)),
     "c3_full": (lambda s: s.workload(3, frames=5)[0],
                 dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7, extractor_exp=2)),
+    # the reference harness's case: blank 1080p frames (every distance ties)
+    # with its BM3Dv2 defaults
+    "blank_full": (lambda s: np.zeros((5, 1080, 1920), np.float32),
+                   dict(sigma=0.7, block_step=4, bm_range=16, radius=2, ps_range=7)),
     # the precision-form workload's shape: 3840x2160, radius 1 (3 frames)
     "c4_full": (lambda s: s.workload(4, frames=3),
                 dict(sigma=5.0, block_step=4, bm_range=16, radius=1, ps_num=2, ps_range=7)),
