@@ -1575,13 +1575,14 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         uint64_t key = sl < staged && sl < 16 ? hk[sl] : kNoKey;
         uint32_t kxy = sl < staged && sl < 16 ? hxy[sl] : 0u;
         wave_lds_sync();
-        if (__builtin_amdgcn_readlane(staged, 0) <= 4 && __builtin_amdgcn_readlane(staged, 32) <= 4) {
+        const int staged0 = __builtin_amdgcn_readlane(staged, 0), staged1 = __builtin_amdgcn_readlane(staged, 32);
+        if (staged0 <= 4 && staged1 <= 4) {
             // the usual case: a 4-lane bitonic sort
             sort16_step<2, 1>(key, kxy, lane);
             sort16_step<4, 2>(key, kxy, lane);
             sort16_step<4, 1>(key, kxy, lane);
-        } else {
-            sort16_dpp(key, kxy, lane);
+        } else if (staged0 <= 16 || staged1 <= 16) {
+            sort16_dpp(key, kxy, lane);  // (both halves past 16: the fallback replaces both)
         }
         if (!(key < kPlaceholderKey)) {
             key = kPlaceholderKey;  // fewer than 8 candidates in the step
