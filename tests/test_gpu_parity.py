@@ -91,6 +91,12 @@ CASES = {
                                   + np.random.default_rng(16).normal(0.0, 3 / 255, (2, 96, 160)),
                                   0.0, 1.0).astype(np.float32),
                 dict(sigma=3.0, block_step=4, bm_range=12, radius=1, ps_num=2, ps_range=4)),
+    # one predictor of 27 x 27 (34 staged columns, more than a 32-lane half
+    # holds): the per-wavefront search takes over
+    "one_wide_predictor_r1": (lambda s: s.shifted_clip(17, 3, 112, 88, 3 / 255, (1.5, -1.0)),
+                              dict(sigma=3.0, block_step=5, bm_range=6, radius=1, ps_num=1, ps_range=13)),
+    "one_wide_predictor_r2": (lambda s: s.shifted_clip(18, 5, 112, 88, 3 / 255, (-1.0, 1.5)),
+                              dict(sigma=3.0, block_step=5, bm_range=6, radius=2, ps_num=1, ps_range=13)),
     # predictor windows too wide for the row-parallel temporal search
     # (61 columns + 7): the per-wavefront search takes over
     "widest_predictive": (lambda s: s.shifted_clip(13, 3, 96, 80, 3 / 255, (1.0, 1.0)),

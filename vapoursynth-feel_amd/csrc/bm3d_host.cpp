@@ -850,8 +850,11 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
             if (rows <= 64 && temporal_lds_bytes(rows, d.ps_range) <= 48 * 1024) {
                 d.t_rows = rows;
             }
-            // both directions in one wavefront: a window row per lane of a half
-            if (d.radius <= 2 && rows <= 32 && predictive_lds_bytes(d.ps_num, d.ps_range) <= 64 * 1024) {
+            // both directions in one wavefront: a window row per lane of a
+            // half, and a window column per lane of a half while staging
+            // (2 ps_range + 8 columns)
+            if (d.radius <= 2 && rows <= 32 && 2 * d.ps_range + 8 <= 32 &&
+                predictive_lds_bytes(d.ps_num, d.ps_range) <= 64 * 1024) {
                 d.pred_pair = 1;
             }
         }
@@ -1213,6 +1216,10 @@ static int check_device_clip(const Instance &d, const vsfeel_bm3d_device_clip *c
     }
     if (clip->pitch < d.width || clip->frame_stride < (int64_t)clip->pitch * (d.height - 1) + d.width) {
         return fail(err, err_len, VSFEEL_BM3D_EINVAL, "device clip pitch or frame stride too small");
+    }
+    // the search kernels address a plane with 32-bit byte offsets
+    if ((int64_t)clip->pitch * d.height * 4 >= (int64_t(1) << 31)) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "device clip pitch too large (a plane must stay below 2 GiB)");
     }
     if (d.final_pass && !clip->ref) {
         return fail(err, err_len, VSFEEL_BM3D_EINVAL, "the final pass needs the ref clip");
