@@ -149,6 +149,25 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
                                void *stream, char *err, size_t err_len);
 
 /*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): vsfeel_bm3d_process_device without a handle, for the
+ * arguments of vsfeel_bm3d_create (radius 0, no ref clip).  It allocates
+ * nothing: its group table, tile tables and tile accumulators live in a
+ * static device array of 64 MiB that each GPU holds until the process ends
+ * (enough for 1920x1080 at block_step 4, bm_range 16); larger frames are
+ * rejected with VSFEEL_BM3D_EINVAL.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (concurrent streams or threads)
+ * race; calls ordered on one stream do not.  Frames [first, first + count)
+ * of the clip are denoised into out like vsfeel_bm3d_process_device, with
+ * the same kernels and the same results.  Returns after enqueueing on stream.
+ */
+int vsfeel_bm3d_process_device_static(const vsfeel_bm3d_args *args, const vsfeel_bm3d_device_clip *clip,
+                                      int32_t first, int32_t count,
+                                      float *out, int32_t out_pitch, int64_t out_frame_stride,
+                                      void *stream, char *err, size_t err_len);
+
+/*
  * Debug: the groups one estimation dispatch of frame m forms (no
  * counterpart buffer in the reference, whose shader keeps them in shared
  * memory, src/bm3d.comp:657-661).  groups receives ref_blocks_y *

@@ -31,6 +31,8 @@ hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
 hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s);
 hipError_t launch_weight_box(const BoxArgs &a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
+hipError_t launch_tile_tables(int *tab, int width, int height, int tiles_x, int tiles_y, int tile_bx, int tile_by,
+                              int nx, int ny, int step, int halo, hipStream_t s);
 
 namespace {
 
@@ -877,6 +879,60 @@ struct vsfeel_bm3d {
 
 using vsfeel::Instance;
 
+// Device memory of vsfeel_bm3d_process_device_static (the entry point that
+// allocates nothing): its group table, the tile reduction's tables and the
+// tile accumulators.  Each GPU the library runs on holds its own copy until
+// the process ends; every call on that GPU reuses it.  Host code uses the
+// per-device address from hipGetSymbolAddress, never this name.
+static constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mu;
+unsigned char *g_static_ptr[kStaticMaxDevices] = {};
+
+// the device address of `scratch` on the current device `dev`, looked up once
+hipError_t static_scratch(int dev, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            return e;
+        }
+        g_static_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_static_ptr[dev];
+    return hipSuccess;
+}
+
+// the most tiles covering one pixel column (or row) of a plane of `size`
+// pixels cut into `tiles` tiles of `tile_blocks` ref blocks
+int max_tile_span(int size, int tiles, int tile_blocks, int nblocks, int step, int halo)
+{
+    int span = 0;
+    for (int p = 0; p < size; ++p) {
+        int first = -1, last = -2;
+        for (int t = 0; t < tiles; ++t) {
+            int lo, hi;
+            vsfeel::tile_span(t, tile_blocks, nblocks, step, size, halo, &lo, &hi);
+            if (lo <= p && p <= hi) {
+                first = first < 0 ? t : first;
+                last = t;
+            }
+        }
+        span = std::max(span, last - first + 1);
+    }
+    return span;
+}
+
+size_t round256(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
 extern "C" {
 
 int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *err, size_t err_len)
@@ -1340,6 +1396,81 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
     // the res slots now hold this call's stacks, not the host path's frames
     std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_process_device_static(const vsfeel_bm3d_args *args, const vsfeel_bm3d_device_clip *clip,
+                                      int32_t first, int32_t count, float *out, int32_t out_pitch,
+                                      int64_t out_frame_stride, void *stream, char *err, size_t err_len)
+{
+    using namespace vsfeel;
+    if (!args) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "null argument");
+    }
+    // the parameters only: no device object of the instance is created, and
+    // device < 0 keeps its destructor from touching the device
+    Instance d;
+    d.device = -1;
+    const std::string msg = parse_args(*args, d);
+    const int dev = d.device;
+    d.device = -1;
+    if (!msg.empty()) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, msg);
+    }
+    if (d.radius != 0 || d.final_pass || !d.tiled || d.short_windows) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL,
+                    "the allocation-free entry point supports radius 0 without a ref clip, on frames whose every "
+                    "search window holds at least 8 candidates");
+    }
+    if (dev < 0 || dev >= kStaticMaxDevices) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "device ordinal out of range");
+    }
+    // the scratch layout: tile tables, group table, tile accumulators
+    const size_t tab_bytes = round256(size_t(2 * d.width + 2 * d.height + d.tiles_x + d.tiles_y) * sizeof(int));
+    const size_t group_bytes = round256(size_t(d.nref) * 8 * sizeof(uint32_t));
+    const size_t acc_elems = size_t(d.tiles_x) * d.tiles_y * 2 * d.region_h * d.region_w;
+    const size_t need = tab_bytes + group_bytes + acc_elems * sizeof(float);
+    if (need > kStaticScratchBytes) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL,
+                    "a " + std::to_string(d.width) + "x" + std::to_string(d.height) + " frame at these parameters needs " +
+                        std::to_string(need) + " bytes of scratch, more than the " +
+                        std::to_string(kStaticScratchBytes) + " the allocation-free entry point holds");
+    }
+    int rc = check_device_clip(d, clip, err, err_len);
+    if (rc != VSFEEL_BM3D_OK) {
+        return rc;
+    }
+    if (first < 0 || count < 0 || first + count > d.nframes) {
+        return fail(err, err_len, VSFEEL_BM3D_ERANGE, "frame range outside the clip");
+    }
+    if (!out || out_pitch < d.width) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "bad output buffer");
+    }
+    VSFEEL_HIP(hipSetDevice(dev), "hipSetDevice");
+    unsigned char *base = nullptr;
+    VSFEEL_HIP(static_scratch(dev, &base), "hipGetSymbolAddress");
+    d.tile_tables = reinterpret_cast<int *>(base);
+    d.groups = reinterpret_cast<uint32_t *>(base + tab_bytes);
+    d.scratch = reinterpret_cast<float *>(base + tab_bytes + group_bytes);
+    d.reduce_span3 = max_tile_span(d.width, d.tiles_x, d.tile_bx, d.nx, d.block_step, d.bm_range) <= 3 &&
+                     max_tile_span(d.height, d.tiles_y, d.tile_by, d.ny, d.block_step, d.bm_range) <= 3 &&
+                     acc_elems < (size_t(1) << 32);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = launch_tile_tables(d.tile_tables, d.width, d.height, d.tiles_x, d.tiles_y, d.tile_bx, d.tile_by,
+                                      d.nx, d.ny, d.block_step, d.bm_range, s);
+    for (int n = first; n < first + count && e == hipSuccess; ++n) {
+        const float *search[kMaxWindow];
+        const float *input[kMaxWindow];
+        device_window(d, clip, n, search, input);
+        // radius 0: the tile reduction writes the frame (fused aggregation);
+        // the frame's own slice is never touched, so no stack is passed
+        e = d.estimate(search, input, clip->pitch, nullptr, d.pitch, s, nullptr, kFilterPlain, Instance::Saved(),
+                       out + (int64_t)(n - first) * out_frame_stride, out_pitch);
+    }
+    d.tile_tables = nullptr;
+    d.groups = nullptr;
+    d.scratch = nullptr;
+    VSFEEL_HIP(e, "launch");
     return VSFEEL_BM3D_OK;
 }
 

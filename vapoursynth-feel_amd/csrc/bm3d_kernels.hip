@@ -2575,6 +2575,46 @@ __global__ __launch_bounds__(256) void fill_zero(float4 *p, long long n4)
     }
 }
 
+// The tile reduction's tables, built on the device: per column, then per
+// row, the first and last tile whose region covers it (an uncovered pixel
+// gets the empty range (0, -1)), then every tile column's and tile row's
+// region origin.  tab holds 2 width + 2 height + tiles_x + tiles_y ints.
+__global__ __launch_bounds__(256) void bm3d_tile_tables(int *tab, int width, int height, int tiles_x, int tiles_y,
+                                                        int tile_bx, int tile_by, int nx, int ny, int step, int halo)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < width + height) {
+        const bool col = i < width;
+        const int p = col ? i : i - width;
+        const int nt = col ? tiles_x : tiles_y;
+        int first = -1, last = -2;
+        for (int t = 0; t < nt; ++t) {
+            int lo, hi;
+            if (col) {
+                tile_span(t, tile_bx, nx, step, width, halo, &lo, &hi);
+            } else {
+                tile_span(t, tile_by, ny, step, height, halo, &lo, &hi);
+            }
+            if (lo <= p && p <= hi) {
+                first = first < 0 ? t : first;
+                last = t;
+            }
+        }
+        int *o = tab + (col ? 2 * p : 2 * width + 2 * p);
+        o[0] = first < 0 ? 0 : first;
+        o[1] = last;
+    } else if (i < width + height + tiles_x + tiles_y) {
+        const int t = i - width - height;
+        int lo, hi;
+        if (t < tiles_x) {
+            tile_span(t, tile_bx, nx, step, width, halo, &lo, &hi);
+        } else {
+            tile_span(t - tiles_x, tile_by, ny, step, height, halo, &lo, &hi);
+        }
+        tab[2 * width + 2 * height + t] = lo;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2721,6 +2761,15 @@ hipError_t launch_fill_zero(float *p, long long n, hipStream_t s)
         blocks = 1;
     }
     hipLaunchKernelGGL(fill_zero, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<float4 *>(p), n4);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_tables(int *tab, int width, int height, int tiles_x, int tiles_y, int tile_bx, int tile_by,
+                              int nx, int ny, int step, int halo, hipStream_t s)
+{
+    const int n = width + height + tiles_x + tiles_y;
+    hipLaunchKernelGGL(bm3d_tile_tables, dim3((n + 255) / 256), dim3(256), 0, s, tab, width, height, tiles_x,
+                       tiles_y, tile_bx, tile_by, nx, ny, step, halo);
     return hipGetLastError();
 }
 

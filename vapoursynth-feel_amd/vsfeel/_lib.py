@@ -32,6 +32,7 @@ EXPORTS = (
     "vsfeel_bm3d_forward_saved",
     "vsfeel_bm3d_backward",
     "vsfeel_bm3d_set_graphs",
+    "vsfeel_bm3d_process_device_static",
 )
 
 KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce",
@@ -137,6 +138,9 @@ def load() -> ctypes.CDLL:
     L.vsfeel_bm3d_process_device.argtypes = [vp, ctypes.POINTER(DeviceClip), i32, i32, vp, i32,
                                              ctypes.c_int64, vp, cp, sz]
     L.vsfeel_bm3d_process_device.restype = i32
+    L.vsfeel_bm3d_process_device_static.argtypes = [ctypes.POINTER(Args), ctypes.POINTER(DeviceClip), i32, i32,
+                                                    vp, i32, ctypes.c_int64, vp, cp, sz]
+    L.vsfeel_bm3d_process_device_static.restype = i32
     L.vsfeel_bm3d_debug_groups.argtypes = [vp, ctypes.POINTER(DeviceClip), i32, vp, vp, cp, sz]
     L.vsfeel_bm3d_debug_groups.restype = i32
     L.vsfeel_bm3d_debug_estimate.argtypes = [vp, ctypes.POINTER(DeviceClip), i32, vp, i32, cp, sz]
