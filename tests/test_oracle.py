@@ -190,6 +190,150 @@ def test_every_group_holds_its_reference_block():
             assert all(0 <= e[2] <= 4 for e in ent)
 
 
+def _d32(cur, plane, left, top, rw, rh):
+    """float32 distances of the sliding metric over a window, written out
+    independently of the oracle: column SSDs v_j = sum_i (cur[i, min(j, 7)]
+    - plane[cy + i, left + j])^2 with even and odd rows in two partial sums
+    (each product and sum rounded to float32, none fused), then
+    D(k) = ((v_k + v_k+1) + (v_k+2 + v_k+3)) + ((v_k+4 + v_k+5) + (v_k+6 + v_k+7))."""
+    ncol = rw + 7
+    a = np.minimum(np.arange(ncol), 7)
+    ref = cur[:, a].astype(np.float32)                                   # [8, ncol]
+    rows = np.lib.stride_tricks.sliding_window_view(plane[top:top + rh + 7, left:left + ncol], 8, axis=0)
+    d = ref[None, :, :] - np.transpose(rows, (0, 2, 1))                  # [rh, 8, ncol] float32
+    e0 = d[:, 0] * d[:, 0]
+    e1 = d[:, 1] * d[:, 1]
+    for i in range(2, 8, 2):
+        e0 = e0 + d[:, i] * d[:, i]
+        e1 = e1 + d[:, i + 1] * d[:, i + 1]
+    v = e0 + e1                                                          # [rh, ncol]
+    k = lambda j: v[:, j:j + rw]
+    return ((k(0) + k(1)) + (k(2) + k(3))) + ((k(4) + k(5)) + (k(6) + k(7)))
+
+
+def _window(cx, cy, rng_, W, H):
+    left, right = max(cx - rng_, 0), min(cx + rng_, W - 8)
+    top, bottom = max(cy - rng_, 0), min(cy + rng_, H - 8)
+    return left, top, right - left + 1, bottom - top + 1
+
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def _smallest8(cands):
+    """The 8 smallest (distance, seq) of [(d, seq, x, y)], placeholders
+    (FLT_MAX, 0, 0) past the last; distances at FLT_MAX never count."""
+    cands = sorted((c for c in cands if c[0] < FLT_MAX), key=lambda c: (c[0], c[1]))[:8]
+    return cands + [(FLT_MAX, 2 ** 31 - 1, 0, 0)] * (8 - len(cands))
+
+
+def _window_cands(cur, plane, left, top, rw, rh, seq_base):
+    d = _d32(cur, plane, left, top, rw, rh)
+    return [(float(d[r, k]), seq_base + r * rw + k, left + k, top + r) for r in range(rh) for k in range(rw)]
+
+
+def _brute_temporal_groups(clip, p, m):
+    """The groups of frame m's estimate, restated from the shader's semantics
+    (src/bm3d.comp:479-663) with global sorts instead of lane lists: the
+    spatial top-8 by (distance, seq); then per direction -1, +1 and step
+    t = 1..r the windows +-ps_range around the first ps_num predictors (the
+    previous list's entries; the spatial top-8 for t = 1), seq = i (2
+    ps_range + 1)^2 + position, the step's top-8 by (distance, seq), its first
+    ps_num inserted into the group with a strict '<' (after equal distances);
+    finally the self-block rule."""
+    W, H, s, R, r = p.width, p.height, p.block_step, p.bm_range, p.radius
+    PN, PR = p.ps_num, p.ps_range
+    win = (2 * PR + 1) ** 2
+    nf = len(clip)
+    frame = lambda z: clip[min(max(m - r + z, 0), nf - 1)]
+    _, nx, ny = po.ref_grid(p)
+    out = np.zeros((ny, nx, 8, 3), np.int32)
+    for by in range(ny):
+        for bx in range(nx):
+            x, y = min(bx * s, W - 8), min(by * s, H - 8)
+            cur = frame(r)[y:y + 8, x:x + 8]
+            sp = _smallest8(_window_cands(cur, frame(r), *_window(x, y, R, W, H), 0))
+            group = [(c[0], c[2], c[3], r) for c in sp]
+            for direction in (-1, 1):
+                preds = [(c[2], c[3]) for c in sp]
+                for t in range(1, r + 1):
+                    z = r + direction * t
+                    cands = []
+                    for i in range(PN):
+                        cands += _window_cands(cur, frame(z), *_window(*preds[i], PR, W, H), i * win)
+                    step = _smallest8(cands)
+                    for i in range(PN):
+                        e = step[i][0]
+                        if e < group[7][0]:
+                            pos = next(q for q in range(8) if e < group[q][0])
+                            group = group[:pos] + [(e, step[i][2], step[i][3], z)] + group[pos:7]
+                    preds = [(c[2], c[3]) for c in step]
+            if not any(g[1] == x and g[2] == y and g[3] == r for g in group):
+                group = [(-1.0, x, y, r)] + group[:7]
+            out[by, bx] = [[g[1], g[2], g[3]] for g in group]
+    return out
+
+
+def test_d32_restatement_equals_the_oracle_distances_bitwise():
+    """The float32 restatement above reproduces the oracle's window
+    distances bit for bit (fp_mode 1 rounds these sums unfused)."""
+    plane = synth.gray_clip(63, 1, 48, 40, 5 / 255)[0]
+    for (x, y, R) in ((17, 13, 6), (0, 0, 5), (40, 32, 4)):
+        d, (left, top) = po.window_dist(plane, x, y, R, x, y, exact=False)
+        mine = _d32(plane[y:y + 8, x:x + 8], plane, left, top, d.shape[1], d.shape[0])
+        assert np.array_equal(d.view(np.uint32), mine.view(np.uint32))
+
+
+# three shifted clips: radius 1 and 2, one to three predictors, windows
+# clamped at the frame edges (small frames, motion up to 2 px per frame)
+TEMPORAL_CLIPS = [
+    ((70, 4, 40, 32, 3 / 255, (1.5, -1.0)), dict(block_step=4, bm_range=5, radius=1, ps_num=1, ps_range=3)),
+    ((71, 5, 44, 36, 3 / 255, (-1.0, 0.5)), dict(block_step=4, bm_range=4, radius=2, ps_num=2, ps_range=2)),
+    ((72, 5, 36, 28, 5 / 255, (2.0, 1.0)), dict(block_step=3, bm_range=6, radius=2, ps_num=3, ps_range=4)),
+]
+
+
+def _periodic_clip(nf, w, h):
+    """Identical noise-free frames of a pattern periodic in x (3 px) and y
+    (2 px), exact in float32: distances tie across positions, windows and
+    directions, so the seq order of the windows and the order of the two
+    directions decide the groups."""
+    y, x = np.mgrid[0:h, 0:w]
+    frame = ((x % 3) + 4 * (y % 2) + 0.5 * ((x // 7) % 2)).astype(np.float32) / 16
+    return np.repeat(frame[None], nf, axis=0)
+
+
+@pytest.mark.parametrize("case", range(len(TEMPORAL_CLIPS) + 1))
+def test_temporal_groups_equal_a_brute_force(case):
+    if case < len(TEMPORAL_CLIPS):
+        (seed, nf, w, h, noise, shift), kw = TEMPORAL_CLIPS[case]
+        clip = synth.shifted_clip(seed, nf, w, h, noise, shift)
+    else:
+        nf, w, h = 5, 36, 30
+        kw = dict(block_step=4, bm_range=4, radius=2, ps_num=2, ps_range=2)
+        clip = _periodic_clip(nf, w, h)
+    p = po.make_params(w, h, **kw)
+    r = kw["radius"]
+    for m in range(nf):
+        window = [clip[min(max(m - r + z, 0), nf - 1)] for z in range(2 * r + 1)]
+        _, g = po.estimate(p, window)
+        want = _brute_temporal_groups(clip, p, m)
+        bad = np.argwhere((g != want).any(axis=(2, 3)))
+        assert len(bad) == 0, f"frame {m}: {len(bad)} groups differ, first {bad[0]}: {g[tuple(bad[0])].tolist()} " \
+                              f"vs {want[tuple(bad[0])].tolist()}"
+
+
+@pytest.mark.parametrize("radius", [1, 2, 3])
+def test_sigma_zero_reproduces_the_clip_at_radius(radius):
+    """sigma = 0 keeps every coefficient: every group reproduces its blocks,
+    weights are equal, and every slice aggregated for frame n holds frame n's
+    own pixels (agg_z), so the output is the input at any radius."""
+    clip = synth.shifted_clip(64 + radius, 2 * radius + 2, 40, 32, 5 / 255, (1.0, -0.5))
+    p = po.make_params(40, 32, sigma=0.0, block_step=4, bm_range=5, radius=radius, ps_num=2, ps_range=3)
+    out = po.clip(p, clip)
+    np.testing.assert_allclose(out, clip, atol=2e-6)
+
+
 # ------------------------------------------------------------- host logic
 def test_ref_grid_counts():
     assert po.ref_grid(po.make_params(640, 360, block_step=8))[0] == 80 * 45
