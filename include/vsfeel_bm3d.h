@@ -70,6 +70,14 @@ typedef struct vsfeel_bm3d_args {
      * the HIP device ordinal (the reference always uses device 0). */
     int32_t fp_mode;
     int32_t device;
+    /* Not in the reference either: the arithmetic of the block-matching
+     * distances.  0: the reference's FP32 sums (plain form, default).
+     * 1 / 2: precision form (radius 0-2), distances of the pixels rounded
+     * once to FP16 / BF16 as norms - 2 dot on MFMA with FP32 accumulation,
+     * a shortlist of twice the entries a search keeps rescored with the
+     * FP32 sums: the groups equal the plain form's wherever its matches lie
+     * in the shortlist. */
+    int32_t precision;
 } vsfeel_bm3d_args;
 
 /* Derived parameters of an instance (what BM3DCreate stores in BM3DData,
@@ -86,6 +94,7 @@ typedef struct vsfeel_bm3d_info {
     int32_t fp_mode;
     int32_t device;
     int32_t pitch;            /* device plane pitch (elements) of the internal buffers */
+    int32_t precision;        /* vsfeel_bm3d_args.precision */
 } vsfeel_bm3d_info;
 
 typedef struct vsfeel_bm3d vsfeel_bm3d;
@@ -229,10 +238,11 @@ int vsfeel_bm3d_backward(vsfeel_bm3d *h, const float *grad_out, int32_t grad_pit
  * events are recorded around every launch on the launch stream.
  * vsfeel_bm3d_set_timing resets the counters; vsfeel_bm3d_kernel_times waits
  * for the recorded launches and returns the summed milliseconds and launch
- * counts per kernel (arrays of 7): [0] slot clear, [1] block matching,
+ * counts per kernel (arrays of 8): [0] slot clear, [1] block matching,
  * [2] collaborative filtering + scatter, [3] aggregation, [4] tile reduction,
  * [5] weight box sums of the other frames' slices (radius > 0), [6] sums of
- * the binned other-frame blocks (radius > 0).
+ * the binned other-frame blocks (radius > 0), [7] plane sets of the
+ * precision form.
  */
 int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
 int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);

@@ -244,3 +244,20 @@ def test_estimate_stack_matches_oracle():
     scale = np.abs(ores).max()
     err = float(np.abs(res - ores).max())
     assert err <= 1e-5 * max(1.0, scale), f"estimate stack differs by {err}"
+
+
+@pytest.mark.parametrize("radius", [1, 2, 3])
+def test_sigma_zero_reproduces_the_clip_at_radius(radius):
+    """sigma = 0 keeps every coefficient, so every slice aggregated for frame
+    n holds frame n's own pixels: the HIP path reproduces a moving clip at
+    radius 1-3 (a slice landing on the wrong frame would not)."""
+    torch = _torch()
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+
+    clip = synth.shifted_clip(64 + radius, 2 * radius + 2, 40, 32, 5 / 255, (1.0, -0.5))
+    nf, h, w = clip.shape
+    print(f"sigma 0 radius {radius} clip={clip.shape}", flush=True)
+    inst = Instance(nf, h, w, sigma=0.0, block_step=4, bm_range=5, radius=radius, ps_num=2, ps_range=3)
+    out = inst.process(torch.from_numpy(clip).to("cuda:0")).cpu().numpy()
+    np.testing.assert_allclose(out, clip, atol=2e-6)

@@ -41,7 +41,61 @@ struct MatchArgs {
     // nonzero: the temporal search runs both directions in one wavefront
     // (bm3d_match_predictive; needs ps_num * (2 ps_range + 1) <= 32)
     int pred_pair;
+    // precision form (operand != kOperandFp32): MFMA distances from the
+    // window frames' plane sets (bm3d_planes, layout below), spatial runs of
+    // mfma_run_blocks ref blocks per workgroup (mfma_runs_x per grid row)
+    int operand;
+    const uint16_t *pset[kMaxWindow];
+    int pset_py;
+    long long pset_elems;
+    int mfma_run_blocks, mfma_runs_x;
+    // nonzero: the temporal search of the precision form runs on MFMA too
+    // (bm3d_match_predictive_mfma: radius <= 2, ps_num <= 8)
+    int mfma_temporal;
 };
+
+// Arithmetic of the block-matching distances: the reference's FP32 sums
+// (plain form), or MFMA dot products of pixels rounded once to FP16 / BF16
+// (precision form; shortlists rescored with the FP32 sums).
+enum OperandType : int {
+    kOperandFp32 = 0,
+    kOperandFp16 = 1,
+    kOperandBf16 = 2,
+};
+
+// Plane set of one search frame for the precision form (bm3d_planes).
+// Every plane is column-major: element (x, y) at x * py + y, py a multiple
+// of 8 and at least height + 24 (rows past the frame hold zeros).
+//   s       the pixels minus kPlaneCentre, rounded once to the operand type
+//   h       sum of s over columns x .. x+7 (float, exact for these operands),
+//           as hi + lo in the operand type (x <= width - 8)
+//   box     sum of s^2 over the 8x8 block at (x, y) (float; x <= width - 8,
+//           y <= height - 8)
+// The 16-bit planes are stored twice, the second copy one row up (its
+// element y holds row y + 1), so that 8 rows from any row start at an even,
+// 4-byte aligned element.  Order: s0 s1 hhi0 hhi1 hlo0 hlo1 (uint16), box
+// (float), each width * py elements.
+constexpr float kPlaneCentre = 0.5f;
+
+__host__ __device__ inline int plane_py(int height) { return (height + 24 + 7) & ~7; }
+
+__host__ __device__ inline long long plane_set_bytes(int width, int height)
+{
+    return (long long)width * plane_py(height) * (6 * 2 + 4);
+}
+
+// LDS of bm3d_match_mfma for a run region of `cols` source columns and a
+// window of `rh` rows: box (rows padded to 16), per wavefront 1 KB (operand
+// rows, shortlist), h hi + lo and s (rows padded to 8, 7 more than the
+// window's: an 8x8 block below its last row)
+constexpr int kMfmaWaveBytes = 1024;
+__host__ __device__ inline int mfma_region_rows(int rh) { return (rh + 7 + 7) & ~7; }
+__host__ __device__ inline int mfma_box_rows(int rh) { return (rh + 15) & ~15; }
+__host__ __device__ inline long long mfma_rows_lds_bytes(int cols, int rh)
+{
+    const long long nhc = cols - 7;
+    return nhc * mfma_box_rows(rh) * 4 + 4LL * kMfmaWaveBytes + (2 * nhc + cols) * mfma_region_rows(rh) * 2LL;
+}
 
 // Dynamic LDS of bm3d_match with the temporal threshold selection: per
 // wavefront (4) a 64-entry stage (8-byte keys, 4-byte positions) and t_rows

@@ -31,6 +31,8 @@ hipError_t launch_fill_zero(float *p, long long n, hipStream_t s);
 hipError_t launch_tile_reduce(const ReduceArgs &a, hipStream_t s);
 hipError_t launch_weight_box(const BoxArgs &a, hipStream_t s);
 hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
+hipError_t launch_planes(const float *src, int pitch, int width, int height, uint16_t *set, int operand,
+                         hipStream_t s);
 hipError_t launch_tile_tables(int *tab, int width, int height, int tiles_x, int tiles_y, int tile_bx, int tile_by,
                               int nx, int ny, int step, int halo, hipStream_t s);
 
@@ -73,6 +75,7 @@ enum KernelKind {
     kKernelReduce,
     kKernelWeightBox,
     kKernelGather,
+    kKernelPlanes,
     kKernelKinds
 };
 
@@ -128,6 +131,12 @@ struct Instance {
     float sigma = 0, sigma_u = 0, sigma_v = 0, extractor = 0;
     int block_step = 8, bm_range = 9, radius = 0, ps_num = 2, ps_range = 4, num_streams = 4;
     int fp_mode = kFpExplicit, device = 0;
+    // distances of the block matching (OperandType): FP32 (plain form) or
+    // MFMA FP16 / BF16 (precision form, with a plane set per window position)
+    int operand = kOperandFp32;
+    uint16_t *psets = nullptr;  // tw plane sets of pset_bytes
+    long long pset_bytes = 0;
+    int mfma_run_blocks = 0, mfma_runs_x = 0;
     int tw = 1, src_ring = 1, res_cap = 1, clips = 1;
     int nx = 0, ny = 0, nref = 0;
     int pitch = 0;     // elements, plane pitch of the internal buffers
@@ -400,6 +409,7 @@ struct Instance {
         (void)hipFree(rec);
         (void)hipFree(cell_count);
         (void)hipFree(cell_list);
+        (void)hipFree(psets);
         if (last_done) {
             (void)hipEventDestroy(last_done);
         }
@@ -476,7 +486,36 @@ struct Instance {
         m.dist_cols = dist_cols;
         m.t_rows = t_rows;
         m.pred_pair = pred_pair;
+        m.operand = operand;
+        if (operand != kOperandFp32) {
+            for (int z = 0; z < tw; ++z) {
+                m.pset[z] = reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(psets) + z * pset_bytes);
+            }
+            m.pset_py = plane_py(height);
+            m.pset_elems = (long long)width * m.pset_py;
+            m.mfma_run_blocks = mfma_run_blocks;
+            m.mfma_runs_x = mfma_runs_x;
+            m.mfma_temporal = radius > 0 && radius <= 2 ? 1 : 0;
+        }
         return m;
+    }
+
+    // block matching of one estimate: in the precision form first the plane
+    // sets of the window frames, then the search
+    hipError_t match(const MatchArgs &m, const float *const *search, int in_pitch, hipStream_t s)
+    {
+        if (operand != kOperandFp32) {
+            for (int z = 0; z < tw; ++z) {
+                uint16_t *set = const_cast<uint16_t *>(m.pset[z]);
+                const hipError_t e = timed_launch(kKernelPlanes, s, [&] {
+                    return launch_planes(search[z], in_pitch, width, height, set, operand, s);
+                });
+                if (e != hipSuccess) {
+                    return e;
+                }
+            }
+        }
+        return timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
     }
 
     using Saved = SavedState;
@@ -523,7 +562,7 @@ struct Instance {
         if (mode != kFilterApply) {
             MatchArgs m = match_args(search, in_pitch, dists_out);
             m.groups = table;
-            e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
+            e = match(m, search, in_pitch, s);
             if (e != hipSuccess) {
                 return e;
             }
@@ -812,6 +851,13 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
     if (d.fp_mode < 0 || d.fp_mode > 2) {
         return "\"fp_mode\" must be 0, 1 or 2";
     }
+    d.operand = a.precision;
+    if (d.operand < kOperandFp32 || d.operand > kOperandBf16) {
+        return "\"precision\" must be 0 (FP32), 1 (FP16 MFMA) or 2 (BF16 MFMA)";
+    }
+    if (d.operand != kOperandFp32 && d.radius >= 3) {
+        return "\"precision\": the MFMA distances serve the sliding metric of radius 0-2";
+    }
     d.device = a.device;
     d.width = f.width;
     d.height = f.height;
@@ -859,6 +905,22 @@ std::string parse_args(const vsfeel_bm3d_args &a, Instance &d)
                 predictive_lds_bytes(d.ps_num, d.ps_range) <= 64 * 1024) {
                 d.pred_pair = 1;
             }
+        }
+    }
+    if (d.operand != kOperandFp32) {
+        // precision form: runs of up to 8 ref blocks whose region fits 64 KB
+        // of LDS; the window's positions must fit the 16-bit seq of its keys
+        const int rh = std::min(2 * d.bm_range + 1, d.height - 7);
+        const int rw = std::min(2 * d.bm_range + 1, d.width - 7);
+        for (int nb = 8; nb >= 1 && d.mfma_run_blocks == 0; --nb) {
+            const int cols = std::min((nb - 1) * d.block_step + 8 + 2 * d.bm_range, d.width);
+            if (mfma_rows_lds_bytes(cols, rh) <= 64 * 1024) {
+                d.mfma_run_blocks = nb;
+                d.mfma_runs_x = (d.nx + nb - 1) / nb;
+            }
+        }
+        if (d.mfma_run_blocks == 0 || d.run_blocks == 0 || (long long)rw * rh >= 65536) {
+            return "\"precision\": bm_range too large for the MFMA search";
         }
     }
     d.tiled = choose_tiles(d.width, d.height, d.block_step, d.bm_range, &d.tile_bx, &d.tile_by, &d.region_w,
@@ -987,6 +1049,12 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         (e = hipMalloc(&d.dst_dev, plane)) != hipSuccess) {
         return dev_fail(e, "hipMalloc");
     }
+    if (d.operand != kOperandFp32) {
+        d.pset_bytes = (plane_set_bytes(d.width, d.height) + 255) & ~255LL;
+        if ((e = hipMalloc(&d.psets, size_t(d.pset_bytes) * d.tw)) != hipSuccess) {
+            return dev_fail(e, "hipMalloc");
+        }
+    }
     if (d.tiled) {
         // which tiles cover each column / row, and every tile's region origin
         std::vector<int> tab(size_t(2) * d.width + 2 * d.height + d.tiles_x + d.tiles_y);
@@ -1105,6 +1173,7 @@ int vsfeel_bm3d_get_info(const vsfeel_bm3d *h, vsfeel_bm3d_info *info)
     info->fp_mode = d.fp_mode;
     info->device = d.device;
     info->pitch = d.pitch;
+    info->precision = d.operand;
     return VSFEEL_BM3D_OK;
 }
 
@@ -1417,10 +1486,10 @@ int vsfeel_bm3d_process_device_static(const vsfeel_bm3d_args *args, const vsfeel
     if (!msg.empty()) {
         return fail(err, err_len, VSFEEL_BM3D_EINVAL, msg);
     }
-    if (d.radius != 0 || d.final_pass || !d.tiled || d.short_windows) {
+    if (d.radius != 0 || d.final_pass || !d.tiled || d.short_windows || d.operand != kOperandFp32) {
         return fail(err, err_len, VSFEEL_BM3D_EINVAL,
-                    "the allocation-free entry point supports radius 0 without a ref clip, on frames whose every "
-                    "search window holds at least 8 candidates");
+                    "the allocation-free entry point supports the plain form at radius 0 without a ref clip, on "
+                    "frames whose every search window holds at least 8 candidates");
     }
     if (dev < 0 || dev >= kStaticMaxDevices) {
         return fail(err, err_len, VSFEEL_BM3D_EINVAL, "device ordinal out of range");
@@ -1600,7 +1669,7 @@ int vsfeel_bm3d_debug_groups(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *clip
     const float *input[kMaxWindow];
     device_window(d, clip, m, search, input);
     const MatchArgs a = d.match_args(search, clip->pitch, d.dists);
-    VSFEEL_HIP(launch_match(a, d.fp_mode, s), "match launch");
+    VSFEEL_HIP(d.match(a, search, clip->pitch, s), "match launch");
     std::vector<uint32_t> packed(size_t(d.nref) * 8);
     VSFEEL_HIP(hipMemcpyAsync(packed.data(), d.groups, packed.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpy");
     if (dists) {

@@ -165,6 +165,7 @@ void VS_CC bm3d_create(const VSMap *in, VSMap *out, void *user_data, VSCore *cor
     array_arg(in, vsapi, "extractor_exp", extractor_exp, &a.extractor_exp, &a.num_extractor_exp);
     a.fp_mode = 1;
     a.device = 0;
+    a.precision = 0;  // the plain FP32 form: the reference's distances
 
     char err[512];
     if (vsfeel_bm3d_create(&a, &d->handle, err, sizeof(err)) != VSFEEL_BM3D_OK) {

@@ -36,7 +36,7 @@ EXPORTS = (
 )
 
 KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce",
-                "bm3d_weight_box", "bm3d_gather")
+                "bm3d_weight_box", "bm3d_gather", "bm3d_planes")
 
 
 class Format(ctypes.Structure):
@@ -71,6 +71,7 @@ class Args(ctypes.Structure):
         ("extractor_exp", _I64), ("num_extractor_exp", ctypes.c_int32),
         ("fp_mode", ctypes.c_int32),
         ("device", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
     ]
 
 
@@ -84,6 +85,7 @@ class Info(ctypes.Structure):
         ("temporal_width", ctypes.c_int32),
         ("ref_blocks_x", ctypes.c_int32), ("ref_blocks_y", ctypes.c_int32),
         ("fp_mode", ctypes.c_int32), ("device", ctypes.c_int32), ("pitch", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
     ]
 
 
@@ -104,6 +106,19 @@ class Saved(ctypes.Structure):
         ("denom", ctypes.c_void_p),
         ("denom_pitch", ctypes.c_int32),
     ]
+
+
+# vsfeel_bm3d_args.precision: the arithmetic of the block-matching distances
+PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
+def precision_code(p) -> int:
+    """0 / 1 / 2 from an int or one of "fp32", "fp16", "bf16"."""
+    if isinstance(p, str):
+        if p not in PRECISION:
+            raise ValueError(f"precision must be one of {sorted(PRECISION)}")
+        return PRECISION[p]
+    return int(p)
 
 
 _lib = None

@@ -157,7 +157,7 @@ class _Bm3dClip(Clip):
 
 def BM3Dv2(clip: Clip, ref: Clip | None = None, sigma=None, block_step=None, bm_range=None,
            radius=None, ps_num=None, ps_range=None, num_streams=None, extractor_exp=None, *,
-           fp_mode: int = 1, device: int = 0) -> Clip:
+           fp_mode: int = 1, device: int = 0, precision="fp32") -> Clip:
     """core.vsfeel.BM3Dv2 (README.md:44-77, src/bm3d.cpp:1558-1574).
 
     sigma (float[], default 3.0), block_step (int[], [1, 8], default 8),
@@ -165,7 +165,9 @@ def BM3Dv2(clip: Clip, ref: Clip | None = None, sigma=None, block_step=None, bm_
     ps_num (int[], [1, 8], default 2), ps_range (int[], > 0, default 4),
     num_streams (int, > 0, default 4), extractor_exp (int, default 0: off).
     ``ref`` switches to the final (Wiener) pass with block matching on ref.
-    fp_mode and device are extensions (multiply-add rounding mode, HIP device).
+    fp_mode and device are extensions (multiply-add rounding mode, HIP device),
+    and so is precision: "fp32" (the reference's distances, default), "fp16" or
+    "bf16" (MFMA distances with an FP32-rescored shortlist, radius 0-2).
     """
     L = _lib.load()
     args = _lib.Args()
@@ -192,6 +194,7 @@ def BM3Dv2(clip: Clip, ref: Clip | None = None, sigma=None, block_step=None, bm_
     put("extractor_exp", _int_array(extractor_exp), ctypes.c_int64)
     args.fp_mode = fp_mode
     args.device = device
+    args.precision = _lib.precision_code(precision)
     handle = ctypes.c_void_p()
     err = _lib.err_buf()
     rc = L.vsfeel_bm3d_create(ctypes.byref(args), ctypes.byref(handle), err, len(err))

@@ -24,7 +24,7 @@ class Instance:
 
     def __init__(self, num_frames: int, height: int, width: int, *, final: bool = False,
                  sigma=3.0, block_step=8, bm_range=9, radius=0, ps_num=2, ps_range=4,
-                 num_streams=4, extractor_exp=0, fp_mode=1, device=0):
+                 num_streams=4, extractor_exp=0, fp_mode=1, device=0, precision="fp32"):
         L = _lib.load()
         fmt = _lib.Format(GRAY, FLOAT, 32, 0, 0, width, height, num_frames)
         a = _lib.Args()
@@ -49,6 +49,7 @@ class Instance:
             put(name, v, ctypes.c_int64, np.int64)
         a.fp_mode = fp_mode
         a.device = device
+        a.precision = _lib.precision_code(precision)
         h = ctypes.c_void_p()
         err = _lib.err_buf()
         rc = L.vsfeel_bm3d_create(ctypes.byref(a), ctypes.byref(h), err, len(err))
@@ -230,13 +231,14 @@ def _instance(x: torch.Tensor, final: bool, kw: dict) -> Instance:
 
 
 def bm3d(x: torch.Tensor, ref: torch.Tensor | None = None, *, sigma=3.0, block_step=8, bm_range=9,
-         radius=0, ps_num=2, ps_range=4, extractor_exp=0, fp_mode=1) -> torch.Tensor:
+         radius=0, ps_num=2, ps_range=4, extractor_exp=0, fp_mode=1, precision="fp32") -> torch.Tensor:
     """Denoise every frame of x (N, H, W); ref selects the Wiener final pass."""
     if x.dim() == 2:
         x = x.unsqueeze(0)
         if ref is not None and ref.dim() == 2:
             ref = ref.unsqueeze(0)
     kw = dict(sigma=sigma, block_step=block_step, bm_range=bm_range, radius=radius,
-              ps_num=ps_num, ps_range=ps_range, extractor_exp=extractor_exp, fp_mode=fp_mode)
+              ps_num=ps_num, ps_range=ps_range, extractor_exp=extractor_exp, fp_mode=fp_mode,
+              precision=precision)
     inst = _instance(x, ref is not None, kw)
     return inst.process(x, ref)
