@@ -63,38 +63,37 @@ enum OperandType : int {
     kOperandBf16 = 2,
 };
 
-// Plane set of one search frame for the precision form (bm3d_planes).
-// Every plane is column-major: element (x, y) at x * py + y, py a multiple
-// of 8 and at least height + 24 (rows past the frame hold zeros).
+// Plane set of one search frame for the precision form (bm3d_planes), with
+// py = plane_py(height) rows per column (rows past the frame hold zeros):
 //   s       the pixels minus kPlaneCentre, rounded once to the operand type
-//   h       sum of s over columns x .. x+7 (float, exact for these operands),
-//           as hi + lo in the operand type (x <= width - 8)
+//           (FP16 or BF16), row-major: (x, y) at y * width + x
+//   h       sum of s over columns x .. x+7 (float, exact for these operands)
+//           as hi + lo in the operand type (x <= width - 8), column-major:
+//           (x, y) at x * py + y
 //   box     sum of s^2 over the 8x8 block at (x, y) (float; x <= width - 8,
-//           y <= height - 8)
-// The 16-bit planes are stored twice, the second copy one row up (its
-// element y holds row y + 1), so that 8 rows from any row start at an even,
-// 4-byte aligned element.  Order: s0 s1 hhi0 hhi1 hlo0 hlo1 (uint16), box
-// (float), each width * py elements.
+//           y <= height - 8), column-major
+// Order: hhi, hlo, s (uint16), box (float), each width * py elements.
 constexpr float kPlaneCentre = 0.5f;
 
 __host__ __device__ inline int plane_py(int height) { return (height + 24 + 7) & ~7; }
 
 __host__ __device__ inline long long plane_set_bytes(int width, int height)
 {
-    return (long long)width * plane_py(height) * (6 * 2 + 4);
+    return (long long)width * plane_py(height) * (3 * 2 + 4);
 }
 
 // LDS of bm3d_match_mfma for a run region of `cols` source columns and a
-// window of `rh` rows: box (rows padded to 16), per wavefront 1 KB (operand
-// rows, shortlist), h hi + lo and s (rows padded to 8, 7 more than the
-// window's: an 8x8 block below its last row)
-constexpr int kMfmaWaveBytes = 1024;
+// window of `rh` rows: box (rows padded to 16), per wavefront 512 bytes
+// (operand rows, shortlist), h hi + lo (column-major, rows padded to 8) and
+// s (row-major), each 7 rows more than the window's
+constexpr int kMfmaWaveBytes = 512;
 __host__ __device__ inline int mfma_region_rows(int rh) { return (rh + 7 + 7) & ~7; }
 __host__ __device__ inline int mfma_box_rows(int rh) { return (rh + 15) & ~15; }
 __host__ __device__ inline long long mfma_rows_lds_bytes(int cols, int rh)
 {
     const long long nhc = cols - 7;
-    return nhc * mfma_box_rows(rh) * 4 + 4LL * kMfmaWaveBytes + (2 * nhc + cols) * mfma_region_rows(rh) * 2LL;
+    return nhc * mfma_box_rows(rh) * 4 + 4LL * kMfmaWaveBytes + 2 * nhc * mfma_region_rows(rh) * 2LL +
+           (rh + 7LL) * cols * 2;
 }
 
 // Dynamic LDS of bm3d_match with the temporal threshold selection: per

@@ -1,12 +1,14 @@
 // Precision form of the BM3Dv2 block matching on gfx950 matrix cores.
 //
-//   bm3d_planes      per search frame: its pixels minus kPlaneCentre rounded
-//                    once to FP16 or BF16 (s), their 8-column sums as hi + lo
-//                    and their 8x8 sums of squares (the plane set of
-//                    bm3d_common.hpp)
-//   bm3d_match_mfma  spatial search of the sliding metric (radius < 3), a run
-//                    of ref blocks per workgroup, one block per wavefront at a
-//                    time
+//   bm3d_planes                 per search frame: its pixels minus kPlaneCentre
+//                               rounded once to FP16 or BF16 (s), their
+//                               8-column sums as hi + lo and their 8x8 sums
+//                               of squares (the plane set of bm3d_common.hpp)
+//   bm3d_match_mfma             spatial search of the sliding metric (radius
+//                               < 3), a run of ref blocks per workgroup, one
+//                               block per wavefront at a time
+//   bm3d_match_predictive_mfma  temporal predictive search (radius 1-2), one
+//                               ref block per wavefront
 //
 // The distance of candidate k of a window row is the reference's sum of 8
 // column SSDs, column c against reference column min(k + c, 7)
@@ -21,10 +23,11 @@
 // with |R_k|^2 = sum_c |R_min(k+c,7)|^2 (the second sum of P only for k < 7).
 // Both sums of P are 16x16 MFMA tiles, window rows as M and window columns
 // as N: the first with a Toeplitz A (row m holds c7 shifted to the row's
-// offset) against 8-row segments of h (hi and lo), the second with Toeplitz
-// rows of every d_j against 8-row segments of s, masked to j >= k.  Products
-// of the rounded operands are exact and accumulate in FP32, so D~ is, up to
-// FP32 rounding, the distance of the rounded blocks.  The 16 smallest D~ (2k
+// offset) against 8-row segments of h (hi and lo); the second with the 8 x 8
+// block of s right of the window's left edge at the row's offset as A (K =
+// 8 rows x 8 columns) against B[(i, j)][k] = d_j[i] for j >= k (j < 7).
+// Products of the rounded operands are exact and accumulate in FP32, so D~
+// is, up to FP32 rounding, the distance of the rounded blocks.  The 16 smallest D~ (2k
 // for the top-8) are rescored with the plain form's FP32 column sums and
 // summation tree and sorted by (distance, seq): the group equals the plain
 // form's wherever the plain top-8 lies in that shortlist.
@@ -97,7 +100,8 @@ __device__ __forceinline__ float from_op<kOperandBf16>(uint32_t h)
     return __uint_as_float(h << 16);
 }
 
-// 8 operands from consecutive 16-bit LDS elements (any even byte address)
+// 8 operands from 8 consecutive 16-bit elements of LDS or global memory at
+// any even byte address (this target's loads need no more alignment)
 __device__ __forceinline__ u32x4 load8(const uint16_t *p)
 {
     u32x4 v;
@@ -111,9 +115,10 @@ __device__ __forceinline__ u32x4 load8(const uint16_t *p)
 // ---------------------------------------------------------------------------
 // Plane set of one search frame (layout in bm3d_common.hpp).  A workgroup
 // makes a 32-column x 64-row tile: the rounded pixels of the tile and its
-// 7-pixel margins in LDS (as floats), the 8-column sums of their squares,
-// then per element (a lane per row, a wavefront per 8 columns) s, h (hi, lo)
-// and box, written to both row copies.  Rows past the frame get zeros.
+// 7-pixel margins in LDS (as floats) and the 8-column sums of their squares;
+// then h (hi, lo) and box column-major (a lane per row, a wavefront per 8
+// columns) and s row-major (a lane per column).  Rows past the frame get
+// zeros.
 // ---------------------------------------------------------------------------
 constexpr int kPlTileX = 32, kPlTileY = 64;
 
@@ -148,9 +153,18 @@ __global__ __launch_bounds__(256) void bm3d_planes(const float *__restrict__ src
     }
     __syncthreads();
     const long long E = (long long)width * py;
-    uint16_t *s0 = set, *s1 = set + E, *hh0 = set + 2 * E, *hh1 = set + 3 * E, *hl0 = set + 4 * E,
-             *hl1 = set + 5 * E;
-    float *box = reinterpret_cast<float *>(set + 6 * E);
+    uint16_t *hhi = set, *hlo = set + E, *srm = set + 2 * E;
+    float *box = reinterpret_cast<float *>(set + 3 * E);
+    {
+        // s, row-major: lanes 0..31 a column each, a wavefront per 16 rows
+        const int c = lane & 31, x = x0 + c;
+        for (int rr = (lane >> 5); rr < kPlTileY / 4; rr += 2) {
+            const int r = wave * (kPlTileY / 4) + rr, y = y0 + r;
+            if (x < width && y < py) {
+                srm[(long long)y * width + x] = uint16_t(to_op<OP>(sv[c][r]));
+            }
+        }
+    }
     const int y = y0 + lane;
     if (y >= py) {
         return;
@@ -161,7 +175,6 @@ __global__ __launch_bounds__(256) void bm3d_planes(const float *__restrict__ src
         if (x >= width) {
             break;
         }
-        const float v = sv[c][lane];
         float h = 0.0f, b = 0.0f;
         if (x <= width - 8 && y < height) {
 #pragma unroll
@@ -175,124 +188,132 @@ __global__ __launch_bounds__(256) void bm3d_planes(const float *__restrict__ src
                 b += sq[c][lane + i];
             }
         }
-        const uint32_t hs = to_op<OP>(v);
         const uint32_t hi = to_op<OP>(h);
         const uint32_t lo = to_op<OP>(h - from_op<OP>(hi));
         const long long o = (long long)x * py + y;
-        s0[o] = uint16_t(hs);
-        hh0[o] = uint16_t(hi);
-        hl0[o] = uint16_t(lo);
+        hhi[o] = uint16_t(hi);
+        hlo[o] = uint16_t(lo);
         box[o] = b;
-        if (y >= 1) {
-            s1[o - 1] = uint16_t(hs);
-            hh1[o - 1] = uint16_t(hi);
-            hl1[o - 1] = uint16_t(lo);
-        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Per-block operands.  pad row j (j < 7: d_j, j = 7: c7) holds the column at
-// elements 16..23 and zeros around it, so that 8 consecutive elements from
-// 16 + o - rho are a Toeplitz row segment.
+// Per-block operands.  The wavefront's scratch holds c7 at elements 16..23
+// of a zero row (so that 8 elements from 16 + o - rho are a Toeplitz row
+// segment) and d_j[i] transposed (row i: j = 0..6, then 0).
 // ---------------------------------------------------------------------------
 constexpr int kPadLen = 48;
-constexpr int kCorrSteps = 6;  // K = 7 columns x 3 segments of 8 rows, 4 segments per MFMA
 
 struct BlockOperands {
-    u32x4 a_main;              // c7 Toeplitz rows for the lane's tile row
-    u32x4 a_corr[kCorrSteps];  // d_j Toeplitz rows, segment 4 step + g
-    float nr_first;            // |R_k|^2 for the lane's column k = n of the first column tile
-    float nr7;                 // |R_k|^2 for k >= 7 (8 |c7|^2)
+    u32x4 a_main;    // -2 c7 Toeplitz row of the lane's tile row
+    u32x4 b_d[2];    // -2 d_j[i] for j >= the lane's column k (K steps of 4 rows i)
+    float nr_first;  // |R_k|^2 for the lane's column k = n of the first column tile
+    float nr7;       // |R_k|^2 for k >= 7 (8 |c7|^2)
 };
 
-// ref: the block's rounded columns (column c at ref[c * cstride], 8 rows)
+// lane i takes lane i + S of its 16-lane row (0 past the row's end)
+template <int S>
+__device__ __forceinline__ float dpp_shl(float v)
+{
+    return __uint_as_float(dpp_mov<0x100 + S>(__float_as_uint(v)));
+}
+
+// ref: the block's rounded pixels, row-major (pixel (c, i) at ref[i * rs + c]).
+// The MFMA operands carry the factor -2 of -2 P (exact: a power of two), so
+// that tiles accumulate onto box + |R_k|^2 and end at D~.
 template <int OP>
-__device__ __forceinline__ BlockOperands block_operands(const uint16_t *ref, int cstride, uint16_t *pad, int lane)
+__device__ __forceinline__ BlockOperands block_operands(const uint16_t *ref, int rs, uint16_t *scratch, int lane)
 {
     const int g = lane >> 4, n = lane & 15;
     const int mrow = lane & 15;
     const int rho = (mrow >> 2) + 4 * (mrow & 3);
-    {
-        const int j = lane >> 3, u = lane & 7;
-        const float c7 = from_op<OP>(ref[7 * cstride + u]);
-        const float cj = from_op<OP>(ref[j * cstride + u]);
-        const uint32_t v = j == 7 ? to_op<OP>(c7) : to_op<OP>(cj - c7);
-        for (int e = lane; e < 8 * kPadLen; e += 64) {
-            pad[e] = 0;
-        }
-        wave_lds_sync();
-        pad[j * kPadLen + 16 + u] = uint16_t(v);
-        wave_lds_sync();
+    uint16_t *c7pad = scratch;                 // [kPadLen]
+    uint16_t *dT = scratch + kPadLen;          // [8 rows i][8 columns j]
+    const int j = lane >> 3, u = lane & 7;     // lane: column j, row u of the block
+    const uint32_t c7h = ref[u * rs + 7];
+    const float c7 = from_op<OP>(c7h);
+    const uint32_t d = j == 7 ? 0u : to_op<OP>(from_op<OP>(ref[u * rs + j]) - c7);
+    // |R_j|^2 with R_j = c7 + d_j (exact in FP32) summed over the 8 rows
+    const float rj = c7 + from_op<OP>(d);
+    float e = rj * rj;
+    e = e + dpp_shl<1>(e);
+    e = e + dpp_shl<2>(e);
+    e = e + dpp_shl<4>(e);
+    if (lane < kPadLen) {
+        c7pad[lane] = 0;
     }
+    wave_lds_sync();
+    if (j == 7) {
+        c7pad[16 + u] = uint16_t(to_op<OP>(-2.0f * c7));
+    }
+    dT[u * 8 + j] = uint16_t(to_op<OP>(-2.0f * from_op<OP>(d)));
+    wave_lds_sync();
     BlockOperands o;
-    // |R_j|^2 in lane j (R_j = c7 + d_j exactly in FP32)
-    float e_own = 0.0f;
-    {
-        const int j = lane & 7;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const float c7 = from_op<OP>(pad[7 * kPadLen + 16 + u]);
-            const float r = j == 7 ? c7 : c7 + from_op<OP>(pad[j * kPadLen + 16 + u]);
-            e_own = __builtin_fmaf(r, r, e_own);
-        }
-    }
     float ev[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        ev[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e_own), j));
+    for (int q = 0; q < 8; ++q) {
+        ev[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), 8 * q));
     }
     o.nr7 = 8.0f * ev[7];
     o.nr_first = o.nr7;
     float suffix = 0.0f;
 #pragma unroll
-    for (int j = 6; j >= 0; --j) {
-        suffix += ev[j];
-        if (n == j) {
-            o.nr_first = suffix + float(j + 1) * ev[7];
+    for (int q = 6; q >= 0; --q) {
+        suffix += ev[q];
+        if (n == q) {
+            o.nr_first = suffix + float(q + 1) * ev[7];
         }
     }
-    o.a_main = load8(pad + 7 * kPadLen + 16 + 8 * g - rho);
+    o.a_main = load8(c7pad + 16 + 8 * g - rho);
+    // B[(i, j)][n] = d_j[i] where j >= n: row i = 4 st + g of dT, the halves
+    // of columns below n cleared
+    u32x4 keep;
 #pragma unroll
-    for (int st = 0; st < kCorrSteps; ++st) {
-        const int c = 4 * st + g;
-        if (c < 21) {
-            const int jj = c / 3, tq = c - 3 * jj;
-            o.a_corr[st] = load8(pad + jj * kPadLen + 16 + 8 * tq - rho);
-        } else {
-            o.a_corr[st] = u32x4{0u, 0u, 0u, 0u};
-        }
+    for (int q = 0; q < 4; ++q) {
+        keep[q] = (2 * q >= n ? 0x0000FFFFu : 0u) | (2 * q + 1 >= n ? 0xFFFF0000u : 0u);
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        o.b_d[st] = load8(dT + (4 * st + g) * 8) & keep;
     }
     return o;
 }
 
-// The keys of one tile's 4 accumulator registers into the lane's ascending
-// 3 smallest: D~ = box + |R_k|^2 - 2 P (clamped at 0), key = its bits with
-// the low bits (mask pm) replaced by seq = seq_base + row * rw + col; candidates outside
-// the window never enter.  Registers whose every row lies past the window
-// are skipped (wave-uniform).
-__device__ __forceinline__ void tile_keys(const f32x4 &acc, const f32x4 &box, float nr, int m0, int g, int col,
-                                          int rw, int rh, uint32_t seq_base, uint32_t pm, uint32_t &m1, uint32_t &m2,
-                                          uint32_t &m3)
+// The keys of one tile's 4 accumulator registers (D~ of the lane's 4
+// candidates: rows m0 + g + 4 r, column col) into the lane's ascending T
+// smallest (T = 2 or 3): the bits of D~ clamped at 0 with the low bits (mask
+// pm) replaced by seq = seq_base + row * rw + col; candidates outside the
+// window (edge tiles only) never enter.  Registers whose every row lies past
+// the window are skipped (rh is wave-uniform).
+template <int T>
+__device__ __forceinline__ void tile_keys(const f32x4 &acc, int m0, int g, int col, int rw, int rh,
+                                          uint32_t seq_base, uint32_t pm, uint32_t &m1, uint32_t &m2, uint32_t &m3)
 {
+    const bool edge = m0 + 16 > rh || col - (col & 15) + 16 > rw;
+    const uint32_t col_out = col < rw ? 0u : ~0u;
+    uint32_t seq = seq_base + uint32_t((m0 + g) * rw + col);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         if (m0 + 4 * r >= rh) {
             break;
         }
-        const int row = m0 + g + 4 * r;
-        const float d = fmaxf(__builtin_fmaf(acc[r], -2.0f, box[r] + nr), 0.0f);
-        uint32_t key = (__float_as_uint(d) & ~pm) | (seq_base + uint32_t(row * rw + col));
-        key = (row < rh && col < rw) ? key : ~0u;
-        m3 = med3_u32(m2, m3, key);
+        // negative D~ (rounding near 0) clamps to +0 as a signed integer
+        uint32_t key = (uint32_t(max(__float_as_int(acc[r]), 0)) & ~pm) | seq;
+        if (edge) {
+            key |= (m0 + g + 4 * r < rh ? col_out : ~0u);
+        }
+        if (T == 3) {
+            m3 = med3_u32(m2, m3, key);
+        }
         m2 = med3_u32(m1, m2, key);
         m1 = min(m1, key);
+        seq += uint32_t(4 * rw);
     }
 }
 
-// The want smallest keys of the lanes' lists (keys are distinct; ~0u never
-// counts): bisection on the key value, then their seqs (key & pm) to
-// stage[0 .. want).  Returns want = min(limit, valid keys).
+// The want smallest keys of the lanes' 3-entry lists (keys are distinct;
+// ~0u never counts): bisection on the key value, then their seqs (key & pm)
+// to stage[0 .. want).  Returns want = min(limit, valid keys).
 __device__ __forceinline__ int shortlist(uint32_t m1, uint32_t m2, uint32_t m3, int limit, uint32_t pm,
                                          uint32_t *stage)
 {
@@ -325,28 +346,23 @@ __device__ __forceinline__ int shortlist(uint32_t m1, uint32_t m2, uint32_t m3, 
     return want;
 }
 
-// lane i takes lane i + S of its 16-lane row (0 past the row's end)
-template <int S>
-__device__ __forceinline__ float dpp_shl(float v)
-{
-    return __uint_as_float(dpp_mov<0x100 + S>(__float_as_uint(v)));
-}
-
 // Exact FP32 distance of one candidate by 8 lanes (the plain form's column
 // SSDs and summation tree, src/bm3d.comp:268-282): lane j = l % 8 takes the
-// SSD of candidate column j (cand: its top-left pixel) against reference
-// column min(k + j, 7) (ref: the reference block's top-left pixel; k the
+// SSD of candidate column j against reference column min(k + j, 7) (k the
 // candidate's position in its window row), then the candidate's 8 lanes add
-// them in the reference's tree; the distance is valid in lane j = 0.
+// them in the reference's tree; the distance is valid in lane j = 0.  ref /
+// cand: the planes (wave-uniform pointers) and the element offsets of the
+// reference block's and the candidate's top-left pixels (planes below 2 GiB).
 template <int CM>
-__device__ __forceinline__ float exact_dist8(const float *ref, const float *cand, int pitch, int k, int jcol)
+__device__ __forceinline__ float exact_dist8(const float *ref, uint32_t ref_off, const float *cand, uint32_t cand_off,
+                                             uint32_t pitch, int k, int jcol)
 {
     const int a = min(k + jcol, 7);
     float cc[8], sv[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        cc[i] = ref[(long long)i * pitch + a];
-        sv[i] = cand[(long long)i * pitch + jcol];
+        cc[i] = ref[ref_off + uint32_t(i) * pitch + uint32_t(a)];
+        sv[i] = cand[cand_off + uint32_t(i) * pitch + uint32_t(jcol)];
     }
     float v = col_ssd<CM>(cc, sv);
     v = v + dpp_shl<1>(v);
@@ -371,15 +387,15 @@ constexpr int kMfmaWaves = 4;
 // ---------------------------------------------------------------------------
 // Spatial search of the precision form (radius < 3).  A workgroup takes a run
 // of mfma_run_blocks consecutive ref blocks of one grid row; their windows
-// share the rows [top, bottom], so the run's region of h (hi, lo) and s is
-// staged once in LDS, column-major with segments of 8 rows starting at the
-// window's top row (16-byte aligned), and box with its rows in the tiles'
-// lane order.  Each wavefront takes one block at a time: its Toeplitz
-// operands, the window's tiles (rows and columns by 16; the correction for
-// k < 7 in the first column tile), each lane's 3 smallest keys, the
-// shortlist of 16, the exact rescoring and a 16-lane sort; then the group
-// (radius 0, with the self-block rule) or the spatial top-8 for the
-// temporal search.
+// share the rows [top, bottom], so the run's region of h (hi, lo; column-
+// major, 8-row segments from the window's top row, 16-byte aligned), box
+// (column-major, rows in the tiles' lane order) and s (row-major) is staged
+// once in LDS.  Each wavefront takes one block at a time: its operands; per
+// band of 16 window rows the correction's A (8 x 8 blocks of s) and up to 3
+// column tiles at once (loads, then MFMAs, then keys); each lane's 3
+// smallest keys; the shortlist of 16, the exact rescoring and a 16-lane
+// sort; then the group (radius 0, with the self-block rule) or the spatial
+// top-8 for the temporal search.
 // ---------------------------------------------------------------------------
 template <int CM, int OP>
 __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
@@ -398,38 +414,39 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     const int rx0 = max(x_first - R, 0);
     const int rx1 = min(x_last + R, W - 8) + 7;
     const int ncols = rx1 - rx0 + 1, nhc = ncols - 7;
-    const int CR = mfma_region_rows(rh), CRB = mfma_box_rows(rh);
+    const int CR = mfma_region_rows(rh), CRB = mfma_box_rows(rh), RSR = rh + 7;
     float *boxr = reinterpret_cast<float *>(mlds);
     unsigned char *wbase = reinterpret_cast<unsigned char *>(boxr + nhc * CRB);
     uint16_t *hhi = reinterpret_cast<uint16_t *>(wbase + kMfmaWaves * kMfmaWaveBytes);
     uint16_t *hlo = hhi + nhc * CR;
-    uint16_t *sr = hlo + nhc * CR;
+    uint16_t *srr = hlo + nhc * CR;  // [RSR][ncols]
     const int z = A.radius;
     const uint16_t *set = A.pset[z];
     const long long E = A.pset_elems;
     const int py = A.pset_py;
-    const int par = top & 1;
     {
-        // h (hi, lo) and s: dword pairs of rows from the row copy whose
-        // element top - par holds row top
+        // h (hi, lo): a column's CR rows from the window's top row, as
+        // dwords (unaligned loads)
         const int dw = CR / 2;
-        const uint32_t *gs = reinterpret_cast<const uint32_t *>(set + par * E);
-        const uint32_t *gh = reinterpret_cast<const uint32_t *>(set + (2 + par) * E);
-        const uint32_t *gl = reinterpret_cast<const uint32_t *>(set + (4 + par) * E);
-        uint32_t *ls = reinterpret_cast<uint32_t *>(sr);
         uint32_t *lh = reinterpret_cast<uint32_t *>(hhi);
         uint32_t *ll = reinterpret_cast<uint32_t *>(hlo);
-        for (int i = threadIdx.x; i < ncols * dw; i += 256) {
+        for (int i = threadIdx.x; i < nhc * dw; i += 256) {
             const int c = i / dw, q = i - c * dw;
-            const long long o = ((long long)(rx0 + c) * py + top - par) / 2 + q;
-            ls[i] = gs[o];
-            if (c < nhc) {
-                lh[i] = gh[o];
-                ll[i] = gl[o];
-            }
+            const long long o = (long long)(rx0 + c) * py + top + 2 * q;
+            uint32_t vh, vl;
+            __builtin_memcpy(&vh, set + o, 4);
+            __builtin_memcpy(&vl, set + E + o, 4);
+            lh[i] = vh;
+            ll[i] = vl;
+        }
+        // s: rows top .. top + RSR - 1, columns rx0 .. rx1
+        const uint16_t *gs = set + 2 * E;
+        for (int i = threadIdx.x; i < RSR * ncols; i += 256) {
+            const int r = i / ncols, c = i - r * ncols;
+            srr[i] = gs[(long long)(top + r) * W + rx0 + c];
         }
         // box: region row 16 b + 4 g + r holds window row 16 b + g + 4 r
-        const float *gb = reinterpret_cast<const float *>(set + 6 * E);
+        const float *gb = reinterpret_cast<const float *>(set + 3 * E);
         for (int i = threadIdx.x; i < nhc * CRB; i += 256) {
             const int c = i / CRB, p = i - c * CRB;
             const int row = (p & ~15) + ((p >> 2) & 3) + 4 * (p & 3);
@@ -439,52 +456,74 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     __syncthreads();
 
     const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
     unsigned char *wl = wbase + wave * kMfmaWaveBytes;
-    uint16_t *pad = reinterpret_cast<uint16_t *>(wl);
-    uint32_t *st_seq = reinterpret_cast<uint32_t *>(wl + 8 * kPadLen * 2);
-    unsigned long long *st_key = reinterpret_cast<unsigned long long *>(wl + 8 * kPadLen * 2 + 64);
-    uint32_t *st_xy = reinterpret_cast<uint32_t *>(wl + 8 * kPadLen * 2 + 64 + 128);
+    uint16_t *scratch = reinterpret_cast<uint16_t *>(wl);                          // 224 bytes
+    uint32_t *st_seq = reinterpret_cast<uint32_t *>(wl + 256);                     // [16]
+    unsigned long long *st_key = reinterpret_cast<unsigned long long *>(wl + 320); // [16]
+    uint32_t *st_xy = reinterpret_cast<uint32_t *>(wl + 448);                      // [16]
     const int g = lane >> 4, n = lane & 15;
+    const int mrow = lane & 15;
+    const int rho = (mrow >> 2) + 4 * (mrow & 3);
     const float *plane = A.plane[z];
     for (int b = wave; b < nb; b += kMfmaWaves) {
         const int bx = bx0 + b;
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         const int rw = right - left + 1;
-        const BlockOperands o = block_operands<OP>(sr + (xb - rx0) * CR + (y - top), CR, pad, lane);
+        const BlockOperands o = block_operands<OP>(srr + (y - top) * ncols + (xb - rx0), ncols, scratch, lane);
         const uint32_t pm = pos_mask(rw * rh);
         uint32_t m1 = ~0u, m2 = ~0u, m3 = ~0u;
+        const uint16_t *s_left = srr + (left - rx0);
         for (int m0 = 0; m0 < rh; m0 += 16) {
-            const int rbase = min(m0 + 8 * g, CR - 8);
-            for (int n0 = 0; n0 < rw; n0 += 16) {
-                const int hc = min(left + n0 + n - rx0, nhc - 1);
-                const u32x4 bh = *reinterpret_cast<const u32x4 *>(hhi + hc * CR + rbase);
-                const u32x4 bl = *reinterpret_cast<const u32x4 *>(hlo + hc * CR + rbase);
-                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                acc = mfma16<OP>(o.a_main, bh, acc);
-                acc = mfma16<OP>(o.a_main, bl, acc);
-                if (n0 == 0) {
+            // the correction's A: 8 x 8 blocks of s at the lane's tile row
+            u32x4 a_s[2];
 #pragma unroll
-                    for (int st = 0; st < kCorrSteps; ++st) {
-                        const int c = 4 * st + g;
-                        const int jj = c / 3, tq = c - 3 * jj;
-                        u32x4 bs = {0u, 0u, 0u, 0u};
-                        if (c < 21 && jj >= n) {
-                            bs = *reinterpret_cast<const u32x4 *>(sr + (left + jj - rx0) * CR + min(m0 + 8 * tq, CR - 8));
-                        }
-                        acc = mfma16<OP>(o.a_corr[st], bs, acc);
+            for (int st = 0; st < 2; ++st) {
+                a_s[st] = load8(s_left + min(m0 + rho + 4 * st + g, RSR - 1) * ncols);
+            }
+            const int rbase = min(m0 + 8 * g, CR - 8);
+            for (int n00 = 0; n00 < rw; n00 += 48) {
+                u32x4 bh[3], bl[3];
+                f32x4 bx4[3], acc[3];
+                int hc[3];
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    hc[t] = min(left + n00 + 16 * t + n - rx0, nhc - 1);
+                    bh[t] = *reinterpret_cast<const u32x4 *>(hhi + hc[t] * CR + rbase);
+                    bl[t] = *reinterpret_cast<const u32x4 *>(hlo + hc[t] * CR + rbase);
+                    bx4[t] = *reinterpret_cast<const f32x4 *>(boxr + hc[t] * CRB + m0 + 4 * g);
+                }
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    // D~ = box + |R_k|^2 - 2 P: the operands carry the -2
+                    const float nr = n00 + 16 * t == 0 ? o.nr_first : o.nr7;
+                    acc[t] = f32x4{bx4[t][0] + nr, bx4[t][1] + nr, bx4[t][2] + nr, bx4[t][3] + nr};
+                    acc[t] = mfma16<OP>(o.a_main, bh[t], acc[t]);
+                }
+                if (n00 == 0) {
+                    acc[0] = mfma16<OP>(a_s[0], o.b_d[0], acc[0]);
+                    acc[0] = mfma16<OP>(a_s[1], o.b_d[1], acc[0]);
+                }
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    acc[t] = mfma16<OP>(o.a_main, bl[t], acc[t]);
+                }
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    const int n0 = n00 + 16 * t;
+                    if (n0 < rw) {
+                        tile_keys<3>(acc[t], m0, g, n0 + n, rw, rh, 0u, pm, m1, m2, m3);
                     }
                 }
-                const f32x4 bx4 = *reinterpret_cast<const f32x4 *>(boxr + hc * CRB + m0 + 4 * g);
-                tile_keys(acc, bx4, n0 == 0 ? o.nr_first : o.nr7, m0, g, n0 + n, rw, rh, 0u, pm, m1, m2, m3);
             }
         }
         const int want = shortlist(m1, m2, m3, 16, pm, st_seq);
         {
             // rescoring, 8 candidates per pass
             const float inv_rw = 1.0f / float(rw);
-            const float *refp = plane + (long long)y * A.pitch + xb;
+            const uint32_t pitch = uint32_t(A.pitch);
+            const uint32_t ref_off = uint32_t(y) * pitch + uint32_t(xb);
             for (int p0 = 0; p0 < want; p0 += 8) {
                 const int ci = p0 + (lane >> 3);
                 const bool on = ci < want;
@@ -492,7 +531,8 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
                 const int row = div_small(sq, inv_rw);
                 const int k = sq - row * rw;
                 const int cx = left + k, cy = top + row;
-                const float d = exact_dist8<CM>(refp, plane + (long long)cy * A.pitch + cx, A.pitch, k, lane & 7);
+                const float d = exact_dist8<CM>(plane, ref_off, plane, uint32_t(cy) * pitch + uint32_t(cx), pitch, k,
+                                                lane & 7);
                 if (on && (lane & 7) == 0) {
                     rescored_key(d, uint32_t(sq), cx, cy, st_key[ci], st_xy[ci]);
                 }
@@ -518,77 +558,98 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     }
 }
 
-// 8 operands (16 bytes) from a plane of the plane set at an even element:
-// a 4-byte aligned load
-__device__ __forceinline__ u32x4 load16g(const uint16_t *p)
-{
-    u32x4 v;
-    __builtin_memcpy(&v, reinterpret_cast<const uint32_t *>(p), 16);
-    return v;
-}
-
-// The keys of one temporal window (left, top, rw x rh candidates, seq base
-// seq_base) into the lane's 3 smallest, its operands read from the frame's
-// plane set in global memory: the row copy of the window's top row parity
-// makes every 8-row segment start at an even element.
+// One 16x16 tile of a temporal window (top-left candidate (left, top), tile
+// rows m0.., columns n0..; wave-uniform geometry), operands straight from
+// the frame's plane set in global memory (unaligned 16-byte loads at 32-bit
+// element offsets from the set): the accumulator ends at D~ of the lane's 4
+// candidates.
 template <int OP>
-__device__ __forceinline__ void window_keys(const BlockOperands &o, const uint16_t *set, long long E, int py, int W,
-                                            int left, int top, int rw, int rh, uint32_t seq_base, uint32_t pm,
-                                            int lane, uint32_t &m1, uint32_t &m2, uint32_t &m3)
+__device__ __forceinline__ f32x4 window_tile(const BlockOperands &o, const uint16_t *set, uint32_t E, uint32_t py,
+                                            uint32_t W, int left, int top, int m0, int n0, int g, int n, int rho)
 {
-    const int g = lane >> 4, n = lane & 15;
-    const int par = top & 1;
-    const uint16_t *sc = set + par * E;
-    const uint16_t *hh = set + (2 + par) * E;
-    const uint16_t *hl = set + (4 + par) * E;
-    const float *box = reinterpret_cast<const float *>(set + 6 * E);
-    for (int m0 = 0; m0 < rh; m0 += 16) {
-        for (int n0 = 0; n0 < rw; n0 += 16) {
-            const int hx = min(left + n0 + n, W - 8);
-            const long long ob = (long long)hx * py + top + m0 + 8 * g - par;
-            const u32x4 bh = load16g(hh + ob);
-            const u32x4 bl = load16g(hl + ob);
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma16<OP>(o.a_main, bh, acc);
-            acc = mfma16<OP>(o.a_main, bl, acc);
-            if (n0 == 0) {
+    const uint32_t hx = uint32_t(min(left + n0 + n, int(W) - 8));
+    const uint32_t oh = hx * py + uint32_t(top + m0 + 8 * g);
+    const u32x4 bh = load8(set + oh);
+    const u32x4 bl = load8(set + E + oh);
+    u32x4 a_s[2];
+    if (n0 == 0) {
 #pragma unroll
-                for (int st = 0; st < kCorrSteps; ++st) {
-                    const int c = 4 * st + g;
-                    const int jj = c / 3, tq = c - 3 * jj;
-                    u32x4 bs = {0u, 0u, 0u, 0u};
-                    if (c < 21 && jj >= n) {
-                        bs = load16g(sc + (long long)(left + jj) * py + top + m0 + 8 * tq - par);
-                    }
-                    acc = mfma16<OP>(o.a_corr[st], bs, acc);
-                }
-            }
-            f32x4 bx4;
-            const float *bp = box + (long long)hx * py + top + m0 + g;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                bx4[r] = bp[4 * r];
-            }
-            tile_keys(acc, bx4, n0 == 0 ? o.nr_first : o.nr7, m0, g, n0 + n, rw, rh, seq_base, pm, m1, m2, m3);
+        for (int st = 0; st < 2; ++st) {
+            a_s[st] = load8(set + 2 * E + uint32_t(top + m0 + rho + 4 * st + g) * W + uint32_t(left));
         }
     }
+    const float *box = reinterpret_cast<const float *>(set + 3 * E);
+    const uint32_t ob = hx * py + uint32_t(top + m0 + g);
+    const float nr = n0 == 0 ? o.nr_first : o.nr7;
+    f32x4 acc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        acc[r] = box[ob + 4u * r] + nr;
+    }
+    acc = mfma16<OP>(o.a_main, bh, acc);
+    if (n0 == 0) {
+        acc = mfma16<OP>(a_s[0], o.b_d[0], acc);
+        acc = mfma16<OP>(a_s[1], o.b_d[1], acc);
+    }
+    return mfma16<OP>(o.a_main, bl, acc);
+}
+
+// lane l's partner l ^ 32 value of v (v_permlane32_swap)
+__device__ __forceinline__ uint32_t partner32(uint32_t v, int lane)
+{
+    const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return lane < 32 ? uint32_t(sw[1]) : uint32_t(sw[0]);
+}
+
+// The SL smallest keys of both directions at once (distinct keys; ~0u never
+// counts): lane l < 32 takes the direction-0 lists (a) of lanes l and
+// l + 32, lane l + 32 the direction-1 lists (b) of both, merged into a
+// sorted 4-entry list; then SL rounds of a half-wave minimum, each half's
+// winner moving on to its next key.  Their seqs go to stage[h][0 .. want[h]).
+__device__ __forceinline__ void extract_both(uint32_t a1, uint32_t a2, uint32_t b1, uint32_t b2, int SL,
+                                             uint32_t pm, uint32_t (*stage)[16], int lane, int (&want)[2])
+{
+    const bool lo = lane < 32;
+    const uint32_t r1 = partner32(lo ? b1 : a1, lane), r2 = partner32(lo ? b2 : a2, lane);
+    const uint32_t o1 = lo ? a1 : b1, o2 = lo ? a2 : b2;
+    // merge of the sorted pairs (o1, o2) and (r1, r2)
+    const uint32_t hi1 = max(o1, r1), lo2 = min(o2, r2);
+    uint32_t m1 = min(o1, r1), m2 = min(hi1, lo2), m3 = max(hi1, lo2), m4 = max(o2, r2);
+    int got = 0;
+    for (int q = 0; q < SL; ++q) {
+        const uint32_t mn = half_min32_u32(m1, lane);
+        const bool valid = mn != ~0u;
+        if (valid && (lane & 31) == 0) {
+            stage[lane >> 5][q] = mn & pm;
+        }
+        got += valid ? 1 : 0;
+        if (m1 == mn) {
+            m1 = m2;
+            m2 = m3;
+            m3 = m4;
+            m4 = ~0u;
+        }
+    }
+    want[0] = __builtin_amdgcn_readlane(got, 0);
+    want[1] = __builtin_amdgcn_readlane(got, 32);
+    wave_lds_sync();
 }
 
 // ---------------------------------------------------------------------------
 // Temporal predictive search of the precision form (radius 1-2,
 // src/bm3d.comp:540-637) after bm3d_match_mfma stored the spatial top-8: one
-// wavefront per ref block.  Each step t and direction h (-1, then +1) tiles
-// the ps_num predictor windows of frame r -+ t (seq base i (2 ps_range + 1)^2
-// for window i) into the lane's 3 smallest keys of that direction; the
-// shortlist of 2 ps_num per direction is rescored exactly and sorted (both
-// directions at once, one 16-lane row each); its first ps_num entries are the
-// next step's predictors and the group's insertions, added afterwards in
-// the shader's order like the plain form's.
+// wavefront per ref block.  Each step t tiles the ps_num predictor windows
+// of both directions (frames r - t and r + t; seq base i (2 ps_range + 1)^2
+// for window i), the two directions' windows side by side, into each lane's
+// 2 smallest keys per direction; the 2 ps_num smallest of each direction
+// are rescored exactly and sorted (one 16-lane row per direction); their
+// first ps_num entries are the next step's predictors and the group's
+// insertions, added afterwards in the shader's order like the plain form's.
 // ---------------------------------------------------------------------------
 template <int CM, int OP>
 __global__ __launch_bounds__(64) void bm3d_match_predictive_mfma(MatchArgs A)
 {
-    __shared__ uint16_t pad[8 * kPadLen];
+    __shared__ uint16_t scratch[112];
     __shared__ uint32_t st_seq[2][16];
     __shared__ unsigned long long st_key[2][16];
     __shared__ uint32_t st_xy[2][16];
@@ -599,17 +660,20 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive_mfma(MatchArgs A)
     }
     const int by = blk / A.nx;
     const int bx = blk - by * A.nx;
-    const int W = A.width, H = A.height, pitch = A.pitch, r = A.radius;
+    const int W = A.width, H = A.height, r = A.radius;
     const int x = min(bx * A.step, W - 8);
     const int y = min(by * A.step, H - 8);
     const int PR = A.ps_range, PN = A.ps_num;
     const int wh = 2 * PR + 1, win = wh * wh;
     const int SL = min(2 * PN, 16);
-    const long long E = A.pset_elems;
-    const int py = A.pset_py;
+    const uint32_t E = uint32_t(A.pset_elems), py = uint32_t(A.pset_py), uW = uint32_t(W);
+    const uint32_t pitch = uint32_t(A.pitch);
     const uint32_t pm = pos_mask(PN * win);
-    const BlockOperands o = block_operands<OP>(A.pset[r] + (long long)x * py + y, py, pad, lane);
-    const float *refp = A.plane[r] + (long long)y * pitch + x;
+    const int g = lane >> 4, n = lane & 15;
+    const int mrow = lane & 15;
+    const int rho = (mrow >> 2) + 4 * (mrow & 3);
+    const BlockOperands o = block_operands<OP>(A.pset[r] + 2 * E + uint32_t(y) * uW + uint32_t(x), W, scratch, lane);
+    const uint32_t ref_off = uint32_t(y) * pitch + uint32_t(x);
 
     // the spatial top-8 starts the group and predicts (lanes h * 16 + i:
     // predictor i of direction h)
@@ -625,21 +689,28 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive_mfma(MatchArgs A)
     uint64_t fkey[2] = {kNoKey, kNoKey};
     uint32_t fxy[2] = {0u, 0u};
     for (int t = 1; t <= r; ++t) {
-        int want[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int z = h ? r + t : r - t;
-            uint32_t m1 = ~0u, m2 = ~0u, m3 = ~0u;
-            for (int p = 0; p < PN; ++p) {
-                const uint32_t q = uint32_t(__shfl(int(pred), h * 16 + p));
-                const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
-                const int left = max(cx - PR, 0), top = max(cy - PR, 0);
-                const int rw = min(cx + PR, W - 8) - left + 1;
-                const int rh = min(cy + PR, H - 8) - top + 1;
-                window_keys<OP>(o, A.pset[z], E, py, W, left, top, rw, rh, uint32_t(p * win), pm, lane, m1, m2, m3);
+        const uint16_t *set0 = A.pset[r - t], *set1 = A.pset[r + t];
+        uint32_t a1 = ~0u, a2 = ~0u, b1 = ~0u, b2 = ~0u, unused = ~0u;
+        for (int p = 0; p < PN; ++p) {
+            // window p of both directions (wave-uniform geometry)
+            const uint32_t q0 = uint32_t(__builtin_amdgcn_readfirstlane(__shfl(int(pred), p)));
+            const uint32_t q1 = uint32_t(__builtin_amdgcn_readfirstlane(__shfl(int(pred), 16 + p)));
+            const int c0 = int(q0 & 0xFFFFu), r0 = int(q0 >> 16), c1 = int(q1 & 0xFFFFu), r1 = int(q1 >> 16);
+            const int l0 = max(c0 - PR, 0), t0 = max(r0 - PR, 0), l1 = max(c1 - PR, 0), t1 = max(r1 - PR, 0);
+            const int w0 = min(c0 + PR, W - 8) - l0 + 1, h0 = min(r0 + PR, H - 8) - t0 + 1;
+            const int w1 = min(c1 + PR, W - 8) - l1 + 1, h1 = min(r1 + PR, H - 8) - t1 + 1;
+            const uint32_t sb = uint32_t(p * win);
+            for (int m0 = 0; m0 < max(h0, h1); m0 += 16) {
+                for (int n0 = 0; n0 < max(w0, w1); n0 += 16) {
+                    const f32x4 acc0 = window_tile<OP>(o, set0, E, py, uW, l0, t0, m0, n0, g, n, rho);
+                    const f32x4 acc1 = window_tile<OP>(o, set1, E, py, uW, l1, t1, m0, n0, g, n, rho);
+                    tile_keys<2>(acc0, m0, g, n0 + n, w0, h0, sb, pm, a1, a2, unused);
+                    tile_keys<2>(acc1, m0, g, n0 + n, w1, h1, sb, pm, b1, b2, unused);
+                }
             }
-            want[h] = shortlist(m1, m2, m3, SL, pm, st_seq[h]);
         }
+        int want[2];
+        extract_both(a1, a2, b1, b2, SL, pm, st_seq, lane, want);
         // rescoring, 8 candidates per pass: slot ci = h * SL + q
         const float inv_win = 1.0f / float(win);
         for (int p0 = 0; p0 < 2 * SL; p0 += 8) {
@@ -656,7 +727,8 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive_mfma(MatchArgs A)
             const int k = local - row * rw;
             const int cx = left + k, cy = top + row;
             const float *plane = A.plane[h ? r + t : r - t];
-            const float d = exact_dist8<CM>(refp, plane + (long long)cy * pitch + cx, pitch, k, lane & 7);
+            const float d = exact_dist8<CM>(A.plane[r], ref_off, plane, uint32_t(cy) * pitch + uint32_t(cx), pitch, k,
+                                            lane & 7);
             if (on && (lane & 7) == 0) {
                 rescored_key(d, uint32_t(sq), cx, cy, st_key[h][q], st_xy[h][q]);
             }
@@ -667,7 +739,13 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive_mfma(MatchArgs A)
         uint64_t key = lane < 32 && qs < want[hs] ? st_key[hs][qs] : kNoKey;
         uint32_t kxy = lane < 32 && qs < want[hs] ? st_xy[hs][qs] : 0u;
         wave_lds_sync();
-        sort16_dpp(key, kxy, lane);
+        if (SL <= 4) {
+            sort16_step<2, 1>(key, kxy, lane);
+            sort16_step<4, 2>(key, kxy, lane);
+            sort16_step<4, 1>(key, kxy, lane);
+        } else {
+            sort16_dpp(key, kxy, lane);
+        }
         if (!(key < kPlaceholderKey)) {
             key = kPlaceholderKey;  // fewer candidates than the step keeps
             kxy = 0u;
