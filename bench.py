@@ -382,27 +382,35 @@ def main():
                                 roofline=roofline(pk5, c5.shape[2], c5.shape[1], kw5, ""))
     if "host_io" in forms:
         from vsfeel import BM3Dv2, Clip
-        node = BM3Dv2(Clip.gray(clip), **kw)
-        node.get_frame(0)
-        h0 = time.perf_counter()
-        for n in range(nf):
-            node.get_frame(n)
-        h1 = time.perf_counter()
-        extra["host_io"] = dict(value=nf / (h1 - h0), unit="frames/s",
+        from vsfeel.bm3d import set_graphs
+        host = {}
+        for graphs in (True, False):
+            node = BM3Dv2(Clip.gray(clip), **kw)
+            set_graphs(node, graphs)
+            node.get_frame(0)
+            h0 = time.perf_counter()
+            for n in range(nf):
+                node.get_frame(n)
+            h1 = time.perf_counter()
+            host[graphs] = nf / (h1 - h0)
+            del node
+        extra["host_io"] = dict(value=host[True], unit="frames/s", direct_launches=host[False],
                                 workload=describe(args.config, clip.shape, kw,
                                                   "host buffers: upload, filter and download per "
-                                                  "frame request, one thread (PCIe-inclusive)"))
-        del node
+                                                  "frame request, one thread (PCIe-inclusive); value "
+                                                  "from per-request graphs, direct_launches without"))
     if "pipe" in forms:
         # the vspipe counterpart with the reference harness's BM3Dv2 defaults
         # (benchmark/bench.py:167-180): 4 threads requesting 1080p frames
         from vsfeel import pipe_bench
         frames = 300
         pipe_fps = pipe_bench.run(frames=frames, synthetic=False, threads=4)
-        extra["pipe"] = dict(value=pipe_fps, unit="frames/s",
+        pipe_direct = pipe_bench.run(frames=frames, synthetic=False, threads=4, graphs=False)
+        extra["pipe"] = dict(value=pipe_fps, unit="frames/s", direct_launches=pipe_direct,
                              workload=f"{frames} frames of 1920x1080 GrayS float32 (16 synthetic frames cycled), "
                                       "host buffers from 4 threads (PCIe-inclusive), sigma=0.7, block_step=4, "
-                                      "bm_range=16, radius=2, ps_range=7, num_streams=4")
+                                      "bm_range=16, radius=2, ps_range=7, num_streams=4; value from "
+                                      "per-request graphs, direct_launches without")
     if "c4" in forms:
         extra["c4"] = bench_c4(max(1, args.steps // 2), 1, max(1, min(16, os.cpu_count() or 1)))
     tsteps = max(1, args.steps // 2)

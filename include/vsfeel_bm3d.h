@@ -248,16 +248,31 @@ int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
 int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);
 
 /*
- * Graph replay of vsfeel_bm3d_process_device (on by default): a call
- * captures its launch sequence (per output frame: slot clears, matching,
- * filtering, the other frames' sums and weights at radius > 0, tile
- * reduction, aggregation) into a HIP graph on the instance's
- * stream, and later calls with identical arguments replay it, ordered after
- * and before the caller's stream by events.  The counterpart of the
- * reference's per-frame command-buffer recording (src/bm3d.cpp:486-623).
- * Timing (vsfeel_bm3d_set_timing) launches directly instead.
+ * Graph replay (on by default).  vsfeel_bm3d_process_device captures its
+ * launch sequence (per output frame: slot clears, matching, filtering, the
+ * other frames' sums and weights at radius > 0, tile reduction,
+ * aggregation) into a HIP graph on the instance's stream, and later calls
+ * with identical arguments replay it, ordered after and before the caller's
+ * stream by events.  vsfeel_bm3d_get_frame launches a request's kernels as
+ * one graph per output slot and recompute pattern (the number of estimates
+ * the request runs), captured on an idle stream and updated in place
+ * (hipGraphExecUpdate) with the request's ring slots and frame.  The
+ * counterpart of the reference's per-frame command-buffer recording
+ * (src/bm3d.cpp:486-623, :899-967).  Timing (vsfeel_bm3d_set_timing)
+ * launches directly instead.
  */
 int vsfeel_bm3d_set_graphs(vsfeel_bm3d *h, int32_t enable);
+
+/*
+ * Test hook, with no counterpart in the reference: the next
+ * vsfeel_bm3d_get_frame request that reaches the stage fails there with
+ * VSFEEL_BM3D_EDEVICE and a "BM3D: ..." message, as a failed upload or
+ * submission fails a getFrame (src/bm3d.cpp:747-764), without touching the
+ * device: stage 1 after the request has staged its frames, before any
+ * upload; stage 2 after its uploads are enqueued, before its kernels; 0
+ * disarms.  Concurrent requests and the next one on the handle complete.
+ */
+int vsfeel_bm3d_fail_next_request(vsfeel_bm3d *h, int32_t stage);
 
 #ifdef __cplusplus
 }

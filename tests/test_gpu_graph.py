@@ -5,7 +5,9 @@ with a given set of arguments and replays it afterwards.  Every replay, with
 the program's own eager work and synchronisations between replays, must give
 bit-identical frames to the directly launched sequence (the accumulation is
 integer fixed point, so the results are deterministic), also after the clip
-memory is rewritten in place and after the output moves.
+memory is rewritten in place and after the output moves.  The host-buffer
+requests (getFrame) launch from per-slot graphs updated per request, and must
+equal direct launches in any order and from several threads.
 """
 
 from __future__ import annotations
@@ -71,3 +73,49 @@ def test_graph_replays_match_direct_launches(name):
     got4 = inst.process(x, first=1, count=nf - 2).cpu().numpy()
     want4 = direct.process(x, first=1, count=nf - 2).cpu().numpy()
     assert np.array_equal(got4, want4), f"{name}: sub-range replay differs"
+
+
+@pytest.mark.parametrize("radius,extractor_exp", [(0, 0), (2, 2)])
+def test_host_requests_from_graphs_match_direct_launches(radius, extractor_exp):
+    # vsfeel_bm3d_get_frame launches a request's kernels as the graph of its
+    # output slot and recompute pattern, updated per request: frames in any
+    # order and from 4 threads equal those of direct launches bit for bit
+    import threading
+
+    _torch()
+    from vsfeel import BM3Dv2, Clip, _lib, synth
+    clip = synth.workload(3, frames=9, width=160, height=96)[0]
+    kw = dict(sigma=3.0, block_step=4, bm_range=8, radius=radius, ps_num=2, ps_range=4,
+              extractor_exp=extractor_exp, num_streams=3)
+    print(f"host graphs radius={radius} clip={clip.shape}", flush=True)
+    L = _lib.load()
+    direct = BM3Dv2(Clip.gray(clip), **kw)
+    assert L.vsfeel_bm3d_set_graphs(direct._handle, 0) == _lib.OK
+    want = [direct.get_frame(n)[0] for n in range(9)]
+
+    node = BM3Dv2(Clip.gray(clip), **kw)
+    for n in [4, 0, 8, 5, 5, 1, 7]:  # seeks (whole windows), repeats, neighbours
+        assert np.array_equal(node.get_frame(n)[0], want[n]), f"graph frame {n} differs"
+    got, errors = {}, []
+
+    def worker(frames):
+        try:
+            for n in frames:
+                got.setdefault(n, []).append(node.get_frame(n)[0])
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    rng = np.random.default_rng(9)
+    threads = [threading.Thread(target=worker, args=(list(rng.permutation(9)),)) for _ in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=100)
+    assert not errors, errors
+    for n, frames in got.items():
+        assert len(frames) == 4
+        for f in frames:
+            assert np.array_equal(f, want[n]), f"graph frame {n} differs under parallel requests"
+    # switching graphs off on a live node drops the slots' graphs
+    assert L.vsfeel_bm3d_set_graphs(node._handle, 0) == _lib.OK
+    assert np.array_equal(node.get_frame(3)[0], want[3])

@@ -106,3 +106,102 @@ def test_parallel_requests_are_deterministic(radius, num_streams, nthreads):
         assert len(frames) == nthreads
         for f in frames:
             assert np.array_equal(f, want[n]), f"frame {n} differs under parallel requests"
+
+
+def _padded(a, fill):
+    """a (h, w) float32 in rows of a 64-byte-rounded stride; the padding holds `fill`."""
+    h, w = a.shape
+    sw = -(-w * 4 // 64) * 16
+    buf = np.full((h, sw), fill, np.float32)
+    buf[:, :w] = a
+    return buf
+
+
+def _raw_frame(node, n, src, ref, dst):
+    """vsfeel_bm3d_get_frame on arbitrary (strided) host planes."""
+    import ctypes
+    from vsfeel import _lib
+    from vsfeel.bm3d import Error
+    L = _lib.load()
+    first, last = ctypes.c_int32(), ctypes.c_int32()
+    assert L.vsfeel_bm3d_frame_window(node._handle, n, ctypes.byref(first), ctypes.byref(last)) == _lib.OK
+    VP = ctypes.c_void_p
+
+    def arrays(planes):
+        fs = range(first.value, last.value + 1)
+        return ((VP * len(fs))(*[planes[f].ctypes.data for f in fs]),
+                (ctypes.c_ssize_t * len(fs))(*[planes[f].strides[0] for f in fs]))
+
+    cp, cs = arrays(src)
+    rp, rs = arrays(ref) if ref is not None else (None, None)
+    err = _lib.err_buf()
+    rc = L.vsfeel_bm3d_get_frame(node._handle, n, cp, cs, rp, rs, dst.ctypes.data, dst.strides[0], err, len(err))
+    if rc != _lib.OK:
+        raise Error(err.value.decode())
+
+
+@pytest.mark.parametrize("final", [False, True])
+def test_padded_strides_leave_destination_padding_untouched(final):
+    # VapourSynth frames have row strides rounded up (src/bm3d.cpp:996-1017
+    # copies rows of width * 4 bytes out of and into them)
+    _torch()
+    from vsfeel import BM3Dv2, Clip, synth
+    nf, h, w = 5, 70, 157
+    clip = synth.gray_clip(21, nf, w, h, 6 / 255)
+    ref = synth.gray_clip(22, nf, w, h, 2 / 255) if final else None
+    kw = dict(sigma=3.0, block_step=4, bm_range=8, radius=1, ps_num=2, ps_range=4, extractor_exp=2)
+    print(f"padded strides final={final} clip={clip.shape}", flush=True)
+    node = BM3Dv2(Clip.gray(clip), Clip.gray(ref) if final else None, **kw)
+    want = [node.get_frame(n)[0] for n in range(nf)]
+    # a fresh node reads the padded planes (nothing cached)
+    node2 = BM3Dv2(Clip.gray(clip), Clip.gray(ref) if final else None, **kw)
+    src = [_padded(clip[f], np.nan)[:, :w] for f in range(nf)]
+    rsrc = [_padded(ref[f], np.nan)[:, :w] for f in range(nf)] if final else None
+    assert src[0].strides[0] % 64 == 0 and src[0].strides[0] > w * 4
+    for n in (2, 0, 4, 1, 3):
+        dbuf = _padded(np.zeros((h, w), np.float32), -7.0)
+        _raw_frame(node2, n, src, rsrc, dbuf[:, :w])
+        assert np.array_equal(dbuf[:, :w], want[n]), f"frame {n} differs with padded strides"
+        assert (dbuf[:, w:] == -7.0).all(), f"frame {n}: destination padding written"
+
+
+@pytest.mark.parametrize("stage", [1, 2])
+def test_failed_request_leaves_the_handle_usable(stage):
+    # a request that fails at its upload (1) or launch (2) returns the
+    # "BM3D: ..." error (src/bm3d.cpp:747-764); concurrent requests complete
+    # and the next request on the handle succeeds
+    _torch()
+    from vsfeel import BM3Dv2, Clip, _lib, synth
+    from vsfeel.bm3d import Error
+    clip = synth.workload(3, frames=8, width=128, height=80)[0]
+    kw = dict(sigma=3.0, block_step=4, bm_range=8, radius=1, ps_num=2, ps_range=4, extractor_exp=2,
+              num_streams=3)
+    print(f"failure hook stage={stage} clip={clip.shape}", flush=True)
+    want = _device_frames(clip, {k: v for k, v in kw.items() if k != "num_streams"})
+    node = BM3Dv2(Clip.gray(clip), **kw)
+    L = _lib.load()
+    assert L.vsfeel_bm3d_fail_next_request(node._handle, 3) == _lib.EINVAL
+    assert L.vsfeel_bm3d_fail_next_request(node._handle, stage) == _lib.OK
+    got, errors = {}, []
+
+    def worker(frames):
+        for n in frames:
+            try:
+                got.setdefault(n, []).append(node.get_frame(n)[0])
+            except Error as e:
+                errors.append((n, str(e)))
+
+    rng = np.random.default_rng(stage)
+    threads = [threading.Thread(target=worker, args=(list(rng.permutation(8)),)) for _ in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=100)
+    assert len(errors) == 1, errors
+    n_bad, msg = errors[0]
+    assert msg.startswith("BM3D: ") and "test hook" in msg, msg
+    assert sum(len(v) for v in got.values()) == 4 * 8 - 1
+    for n, frames in got.items():
+        for f in frames:
+            assert np.array_equal(f, want[n]), f"frame {n} differs next to a failed request"
+    assert np.array_equal(node.get_frame(n_bad)[0], want[n_bad])

@@ -12,10 +12,10 @@ def test_defaults_are_the_reference_harness_defaults(capsys):
     seen = {}
 
     def fake_run(frames, synthetic, threads, width, height, sigma, radius, bm_range, ps_range, block_step,
-                 num_streams):
+                 num_streams, graphs):
         seen.update(frames=frames, synthetic=synthetic, width=width, height=height, sigma=sigma,
                     radius=radius, bm_range=bm_range, ps_range=ps_range, block_step=block_step,
-                    num_streams=num_streams)
+                    num_streams=num_streams, graphs=graphs)
         return 50.0
 
     orig = pipe_bench.run
@@ -26,7 +26,7 @@ def test_defaults_are_the_reference_harness_defaults(capsys):
         pipe_bench.run = orig
     # BM3Dv2 defaults of the reference harness (benchmark/bench.py:167-180)
     assert seen == dict(frames=1000, synthetic=True, width=1920, height=1080, sigma=0.7, radius=2,
-                        bm_range=16, ps_range=7, block_step=4, num_streams=4)
+                        bm_range=16, ps_range=7, block_step=4, num_streams=4, graphs=True)
     assert capsys.readouterr().out.strip() == "Output 1000 frames in 20.00 seconds (50.00 fps)"
 
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -78,6 +79,9 @@ enum KernelKind {
     kKernelPlanes,
     kKernelKinds
 };
+
+// stages of a host-buffer request that the test hook can fail
+enum FailStage { kFailNone = 0, kFailUpload = 1, kFailLaunch = 2 };
 
 // LDS bytes a tile accumulator may take (one 512-thread workgroup per CU,
 // 16 bytes per region pixel), and the groups one pass of it filters
@@ -200,18 +204,18 @@ struct Instance {
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;  // orders the instance stream with a null-stream caller
 
     // The host-buffer path (allocated on first use).  Requests overlap: a
-    // request enqueues its uploads on copy_in, its kernels on `stream` and
-    // its download on copy_out under the mutex, then waits for its download
-    // and copies the frame out without it, while later requests proceed.
-    // Each source ring slot has the event after the last kernels that read
-    // it, which an upload into the slot waits for; each request takes one of
-    // num_streams output slots (a device plane, a pinned download plane).
-    // Two pinned upload planes are used in turn, each with the event of its
-    // last copy.  All planes have the library's pitch.
+    // request takes one of num_streams output slots (a device plane, a
+    // pinned download plane, pinned staging planes, its graphs) under the
+    // mutex, copies the frames the ring lacks into the slot's staging planes
+    // without it (src/bm3d.cpp:772-814 stages into the stream's own mapped
+    // buffer the same way), then under the mutex again enqueues its uploads
+    // on copy_in, its kernels on `stream` and its download on copy_out,
+    // waits for its download and copies the frame out without it.  Each
+    // source ring slot has the event after the last kernels that read it,
+    // which an upload into the slot waits for.  All planes have the
+    // library's pitch.
     hipStream_t copy_in = nullptr, copy_out = nullptr;
-    float *pin_up[2] = {nullptr, nullptr};
-    hipEvent_t pin_ev[2] = {nullptr, nullptr};
-    int pin_next = 0;
+    hipStream_t capture = nullptr;       // idle stream the per-request graphs are captured on
     hipEvent_t up_ev = nullptr;          // a request's uploads done (copy_in)
     std::vector<hipEvent_t> slot_ev;     // per src ring slot: after its last readers (stream)
     std::vector<char> slot_ev_set;
@@ -221,10 +225,22 @@ struct Instance {
         hipEvent_t done = nullptr;       // kernels done (stream)
         hipEvent_t down = nullptr;       // download done (copy_out)
         bool busy = false, used = false;
+        // pinned staging planes, `clips` per staged frame ([ref][source] like
+        // a ring slot), grown to the most frames one request has staged; the
+        // slot's previous request waited for its download, which follows its
+        // uploads, so the planes are free while the slot is held
+        std::vector<float *> stage;
+        // graph of the request's kernels by the number of estimates it runs
+        // (its recompute pattern), updated in place per request: a slot's
+        // graph never runs while a request holds the slot
+        std::vector<hipGraphExec_t> graph;
     };
     std::vector<OutSlot> outs;
     int out_next = 0;
     std::condition_variable out_cv;
+    // test hook: the next request that reaches the stage fails there
+    // (kFailUpload, kFailLaunch), without touching the device
+    std::atomic<int> fail_next{0};
 
     hipError_t pin_alloc()
     {
@@ -237,13 +253,10 @@ struct Instance {
             e = hipStreamCreateWithFlags(&copy_out, hipStreamNonBlocking);
         }
         if (e == hipSuccess) {
-            e = hipEventCreateWithFlags(&up_ev, hipEventDisableTiming);
+            e = hipStreamCreateWithFlags(&capture, hipStreamNonBlocking);
         }
-        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-            e = hipHostMalloc(reinterpret_cast<void **>(&pin_up[k]), bytes, hipHostMallocDefault);
-            if (e == hipSuccess) {
-                e = hipEventCreateWithFlags(&pin_ev[k], hipEventDisableTiming);
-            }
+        if (e == hipSuccess) {
+            e = hipEventCreateWithFlags(&up_ev, hipEventDisableTiming);
         }
         slot_ev.assign(size_t(src_ring), nullptr);
         slot_ev_set.assign(size_t(src_ring), 0);
@@ -252,6 +265,7 @@ struct Instance {
         }
         std::vector<OutSlot> o(size_t(std::max(num_streams, 1)));
         for (OutSlot &q : o) {
+            q.graph.assign(size_t(tw) + 1, nullptr);
             if (e == hipSuccess) {
                 e = hipMalloc(&q.dev, bytes);
             }
@@ -272,12 +286,26 @@ struct Instance {
         return e;
     }
 
+    void drop_slot_graphs(OutSlot &q)
+    {
+        for (hipGraphExec_t &g : q.graph) {
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        }
+    }
+
     void free_outs()
     {
         for (OutSlot &q : outs) {
+            drop_slot_graphs(q);
             (void)hipFree(q.dev);
             if (q.pin) {
                 (void)hipHostFree(q.pin);
+            }
+            for (float *p : q.stage) {
+                (void)hipHostFree(p);
             }
             if (q.done) {
                 (void)hipEventDestroy(q.done);
@@ -289,26 +317,30 @@ struct Instance {
         outs.clear();
     }
 
-    // copies a host plane (row stride in bytes) to a device plane of the
-    // library's pitch: rows into a pinned plane, then one DMA copy
-    hipError_t upload(float *dev, const float *src, ptrdiff_t stride, hipStream_t s)
+    // copies the host planes of one frame (row strides in bytes) into
+    // staging planes k of slot q, growing them as needed
+    hipError_t stage(OutSlot &q, int k, const float *src, ptrdiff_t stride, const float *ref, ptrdiff_t ref_stride)
     {
-        const int k = pin_next;
-        pin_next ^= 1;
-        hipError_t e = hipEventSynchronize(pin_ev[k]);  // its previous copy has left the buffer
-        if (e != hipSuccess) {
-            return e;
+        const size_t need = size_t(k + 1) * size_t(clips);
+        while (q.stage.size() < need) {
+            float *p = nullptr;
+            const hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), size_t(pe) * sizeof(float),
+                                               hipHostMallocDefault);
+            if (e != hipSuccess) {
+                return e;
+            }
+            q.stage.push_back(p);
         }
         const size_t row_bytes = size_t(width) * sizeof(float);
-        for (int y = 0; y < height; ++y) {
-            std::memcpy(pin_up[k] + (size_t)y * pitch, reinterpret_cast<const char *>(src) + (ptrdiff_t)y * stride,
-                        row_bytes);
+        for (int c = 0; c < clips; ++c) {
+            const char *from = reinterpret_cast<const char *>(c == clips - 1 ? src : ref);
+            const ptrdiff_t st = c == clips - 1 ? stride : ref_stride;
+            float *to = q.stage[size_t(k) * clips + c];
+            for (int y = 0; y < height; ++y) {
+                std::memcpy(to + (size_t)y * pitch, from + (ptrdiff_t)y * st, row_bytes);
+            }
         }
-        e = hipMemcpyAsync(dev, pin_up[k], size_t(pe) * sizeof(float), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) {
-            e = hipEventRecord(pin_ev[k], s);
-        }
-        return e;
+        return hipSuccess;
     }
 
     // optional per-kernel timing: HIP events recorded around every launch on
@@ -416,14 +448,6 @@ struct Instance {
         if (graph_exec) {
             (void)hipGraphExecDestroy(graph_exec);
         }
-        for (int k = 0; k < 2; ++k) {
-            if (pin_up[k]) {
-                (void)hipHostFree(pin_up[k]);
-            }
-            if (pin_ev[k]) {
-                (void)hipEventDestroy(pin_ev[k]);
-            }
-        }
         free_outs();
         for (hipEvent_t e : slot_ev) {
             if (e) {
@@ -433,7 +457,7 @@ struct Instance {
         if (up_ev) {
             (void)hipEventDestroy(up_ev);
         }
-        for (hipStream_t q : {copy_in, copy_out}) {
+        for (hipStream_t q : {copy_in, copy_out, capture}) {
             if (q) {
                 (void)hipStreamDestroy(q);
             }
@@ -1209,7 +1233,6 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
     }
     std::unique_lock<std::mutex> lock(d.mu);
     VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
-    hipStream_t s = d.stream;
     VSFEEL_HIP(d.pin_alloc(), "hipHostMalloc");
     // an output slot: the request owns it until its frame is copied out
     d.out_cv.wait(lock, [&] { return !d.outs[size_t(d.out_next)].busy; });
@@ -1231,6 +1254,39 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         }
     } release{d, o, lock};
 
+    const int r = d.radius, nf = d.nframes;
+    const int lo = clampi(n - 2 * r, 0, nf - 1), hi = clampi(n + 2 * r, 0, nf - 1);
+    auto missing = [&](int f) {
+        return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing plane of frame " + std::to_string(f));
+    };
+
+    // stage the window frames the ring lacks now into the slot's pinned
+    // planes, without the mutex; at radius 0 the frame is uploaded for every
+    // request (the reference's per-stream slot)
+    int stage_of[4 * kMaxRadius + 1];
+    int staged = 0;
+    for (int f = lo; f <= hi; ++f) {
+        const bool need = r == 0 || d.src_frame[f % d.src_ring] != f;
+        stage_of[f - lo] = need ? staged++ : -1;
+    }
+    lock.unlock();
+    for (int f = lo; f <= hi; ++f) {
+        const int i = f - lo;
+        if (stage_of[i] < 0) {
+            continue;
+        }
+        if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
+            return missing(f);
+        }
+        VSFEEL_HIP(d.stage(o, stage_of[i], clip_planes[i], clip_strides[i], d.final_pass ? ref_planes[i] : nullptr,
+                           d.final_pass ? ref_strides[i] : 0),
+                   "hipHostMalloc");
+    }
+    if (d.fail_next.load() == kFailUpload && d.fail_next.exchange(kFailNone) == kFailUpload) {
+        return fail(err, err_len, VSFEEL_BM3D_EDEVICE, "upload of frame " + std::to_string(n) + " failed (test hook)");
+    }
+    lock.lock();
+
     // a request alone (every earlier one has copied its frame out, so their
     // work is done) keeps its copies on the kernel stream: no cross-stream
     // waits to pay when nothing can overlap
@@ -1239,44 +1295,55 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         in_flight += q.busy ? 1 : 0;
     }
     const bool alone = in_flight == 1;
+    hipStream_t s = d.stream;
     hipStream_t cin = alone ? s : d.copy_in, cout = alone ? s : d.copy_out;
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
-    const int r = d.radius, nf = d.nframes;
-    const int lo = clampi(n - 2 * r, 0, nf - 1), hi = clampi(n + 2 * r, 0, nf - 1);
-    const size_t row_bytes = size_t(d.width) * sizeof(float);
 
-    // upload the window frames the ring does not hold yet (src/bm3d.cpp:772-814)
-    // on the copy stream, each after the kernels that last read its slot; at
-    // radius 0 the frame is uploaded for every request (the reference's
-    // per-stream slot), into the next ring slot so that it overlaps earlier
-    // requests
+    // upload the window frames the ring does not hold (src/bm3d.cpp:772-814)
+    // on the copy stream, each after the kernels that last read its slot;
+    // requests that ran while this one staged may have filled a slot (its
+    // staged copy goes unused) or evicted a frame (staged now, under the mutex)
     for (int f = lo; f <= hi; ++f) {
         const int slot = f % d.src_ring;
         if (r > 0 && d.src_frame[slot] == f) {
             continue;
         }
         const int i = f - lo;
-        if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
-            return fail(err, err_len, VSFEEL_BM3D_EINVAL, "missing plane of frame " + std::to_string(f));
+        if (stage_of[i] < 0) {
+            if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
+                return missing(f);
+            }
+            stage_of[i] = staged++;
+            VSFEEL_HIP(d.stage(o, stage_of[i], clip_planes[i], clip_strides[i],
+                               d.final_pass ? ref_planes[i] : nullptr, d.final_pass ? ref_strides[i] : 0),
+                       "hipHostMalloc");
         }
         if (!alone && d.slot_ev_set[size_t(slot)]) {
             VSFEEL_HIP(hipStreamWaitEvent(cin, d.slot_ev[size_t(slot)], 0), "hipStreamWaitEvent");
         }
         d.src_frame[slot] = -1;  // until the upload is enqueued
         // in the final pass the slot holds [ref][source] like the reference's ring
-        VSFEEL_HIP(d.upload(d.src_slot(slot, d.clips - 1), clip_planes[i], clip_strides[i], cin), "upload");
-        if (d.final_pass) {
-            VSFEEL_HIP(d.upload(d.src_slot(slot, 0), ref_planes[i], ref_strides[i], cin), "upload");
+        for (int c = 0; c < d.clips; ++c) {
+            VSFEEL_HIP(hipMemcpyAsync(d.src_slot(slot, c), o.stage[size_t(stage_of[i]) * d.clips + c],
+                                      size_t(d.pe) * sizeof(float), hipMemcpyHostToDevice, cin),
+                       "hipMemcpyAsync");
         }
         d.src_frame[slot] = f;
     }
     if (!alone) {
         VSFEEL_HIP(hipEventRecord(d.up_ev, cin), "hipEventRecord");
         VSFEEL_HIP(hipStreamWaitEvent(s, d.up_ev, 0), "hipStreamWaitEvent");
+        // the slot's previous frame may still be on its way out (the fused
+        // reduction or the aggregation writes it)
+        if (o.used) {
+            VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+        }
     }
 
-    // estimate stacks of the window frames (src/bm3d.cpp:558-602)
+    // estimate stacks of the window frames the cache lacks (src/bm3d.cpp:558-602)
     std::vector<int> slot_of(d.tw);
+    int est_i[kMaxWindow], est_m[kMaxWindow];
+    int nest = 0;
     for (int i = 0; i < d.tw; ++i) {
         const int m = clampi(n - r + i, 0, nf - 1);
         const int slot = r == 0 ? 0 : m % d.res_cap;
@@ -1284,29 +1351,87 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         if (r > 0 && d.res_frame[slot] == m) {
             continue;
         }
-        const float *search[kMaxWindow];
-        const float *input[kMaxWindow];
-        for (int z = 0; z < d.tw; ++z) {
-            const int ring = clampi(m - r + z, 0, nf - 1) % d.src_ring;
-            search[z] = d.src_slot(ring, 0);
-            input[z] = d.src_slot(ring, d.clips - 1);
+        // the same frame twice in the window (clamped at the clip's ends)
+        bool dup = false;
+        for (int k = 0; k < nest; ++k) {
+            dup = dup || est_m[k] == m;
         }
-        // the slot's previous frame may still be on its way out (the
-        // fused reduction writes it)
-        if (o.used && !alone && d.fuses()) {
-            VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+        if (!dup) {
+            est_i[nest] = i;
+            est_m[nest] = m;
+            ++nest;
         }
-        VSFEEL_HIP(d.estimate(search, input, d.pitch, d.res_slot(slot), d.pitch, s, nullptr, kFilterPlain,
-                              Instance::Saved(), o.dev, d.pitch),
-                   "estimate launch");
-        d.res_frame[slot] = r == 0 ? -1 : m;
     }
-    if (!d.fuses()) {
-        // the slot's previous frame may still be on its way out
-        if (o.used && !alone) {
-            VSFEEL_HIP(hipStreamWaitEvent(s, o.down, 0), "hipStreamWaitEvent");
+    auto enqueue = [&](hipStream_t q) -> hipError_t {
+        for (int k = 0; k < nest; ++k) {
+            const int m = est_m[k];
+            const float *search[kMaxWindow];
+            const float *input[kMaxWindow];
+            for (int z = 0; z < d.tw; ++z) {
+                const int ring = clampi(m - r + z, 0, nf - 1) % d.src_ring;
+                search[z] = d.src_slot(ring, 0);
+                input[z] = d.src_slot(ring, d.clips - 1);
+            }
+            const hipError_t e = d.estimate(search, input, d.pitch, d.res_slot(slot_of[est_i[k]]), d.pitch, q,
+                                            nullptr, kFilterPlain, Instance::Saved(), o.dev, d.pitch);
+            if (e != hipSuccess) {
+                return e;
+            }
         }
-        VSFEEL_HIP(d.aggregate(n, slot_of, o.dev, d.pitch, s), "aggregate launch");
+        return d.fuses() ? hipSuccess : d.aggregate(n, slot_of, o.dev, d.pitch, q);
+    };
+    if (d.fail_next.load() == kFailLaunch && d.fail_next.exchange(kFailNone) == kFailLaunch) {
+        // the uploads stay enqueued: their slots are ordered before later
+        // uploads into them as after kernels
+        for (int f = lo; f <= hi; ++f) {
+            const int slot = f % d.src_ring;
+            VSFEEL_HIP(hipEventRecord(d.slot_ev[size_t(slot)], s), "hipEventRecord");
+            d.slot_ev_set[size_t(slot)] = 1;
+        }
+        VSFEEL_HIP(d.end_on(s), "hipEventRecord");
+        return fail(err, err_len, VSFEEL_BM3D_EDEVICE, "launch for frame " + std::to_string(n) + " failed (test hook)");
+    }
+    if (d.graphs && !d.timing) {
+        // the kernels from the slot's graph for this recompute pattern: captured
+        // on the idle capture stream, then updated into the slot's executable
+        // graph (whose last launch has finished: the slot's previous request
+        // waited for its download)
+        VSFEEL_HIP(hipStreamBeginCapture(d.capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const hipError_t e = enqueue(d.capture);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(d.capture, &g);
+        if (e != hipSuccess || ec != hipSuccess) {
+            if (g) {
+                (void)hipGraphDestroy(g);
+            }
+            VSFEEL_HIP(e, "launch during graph capture");
+            VSFEEL_HIP(ec, "hipStreamEndCapture");
+        }
+        hipGraphExec_t &x = o.graph[size_t(nest)];
+        if (x) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+            if (hipGraphExecUpdate(x, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(x);
+                x = nullptr;
+            }
+        }
+        hipError_t ei = hipSuccess;
+        if (!x) {
+            ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        }
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            x = nullptr;
+            VSFEEL_HIP(ei, "hipGraphInstantiate");
+        }
+        VSFEEL_HIP(hipGraphLaunch(x, s), "hipGraphLaunch");
+    } else {
+        VSFEEL_HIP(enqueue(s), "launch");
+    }
+    for (int k = 0; k < nest; ++k) {
+        d.res_frame[size_t(slot_of[est_i[k]])] = r == 0 ? -1 : est_m[k];
     }
     VSFEEL_HIP(hipEventRecord(o.done, s), "hipEventRecord");
     for (int f = lo; f <= hi; ++f) {
@@ -1326,10 +1451,20 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
     // the frame leaves the slot without the lock: later requests proceed
     lock.unlock();
     VSFEEL_HIP(hipEventSynchronize(o.down), "hipEventSynchronize");
+    const size_t row_bytes = size_t(d.width) * sizeof(float);
     for (int y = 0; y < d.height; ++y) {
         std::memcpy(reinterpret_cast<char *>(dst) + (ptrdiff_t)y * dst_stride, o.pin + (size_t)y * d.pitch,
                     row_bytes);
     }
+    return VSFEEL_BM3D_OK;
+}
+
+int vsfeel_bm3d_fail_next_request(vsfeel_bm3d *h, int32_t stage)
+{
+    if (!h || stage < vsfeel::kFailNone || stage > vsfeel::kFailLaunch) {
+        return VSFEEL_BM3D_EINVAL;
+    }
+    h->inst.fail_next.store(stage);
     return VSFEEL_BM3D_OK;
 }
 
@@ -1745,15 +1880,22 @@ int vsfeel_bm3d_set_graphs(vsfeel_bm3d *h, int32_t enable)
     Instance &d = h->inst;
     std::lock_guard<std::mutex> lock(d.mu);
     d.graphs = enable != 0;
-    if (!d.graphs && d.graph_exec) {
+    if (!d.graphs) {
         if (d.device >= 0) {
             (void)hipSetDevice(d.device);
         }
         if (d.stream) {
             (void)hipStreamSynchronize(d.stream);
         }
-        (void)hipGraphExecDestroy(d.graph_exec);
-        d.graph_exec = nullptr;
+        if (d.graph_exec) {
+            (void)hipGraphExecDestroy(d.graph_exec);
+            d.graph_exec = nullptr;
+        }
+        // requests that hold a slot launched their graph before it was
+        // synchronized above
+        for (Instance::OutSlot &q : d.outs) {
+            d.drop_slot_graphs(q);
+        }
     }
     return VSFEEL_BM3D_OK;
 }

@@ -33,6 +33,7 @@ EXPORTS = (
     "vsfeel_bm3d_backward",
     "vsfeel_bm3d_set_graphs",
     "vsfeel_bm3d_process_device_static",
+    "vsfeel_bm3d_fail_next_request",
 )
 
 KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce",
@@ -164,6 +165,8 @@ def load() -> ctypes.CDLL:
     L.vsfeel_bm3d_set_timing.restype = i32
     L.vsfeel_bm3d_set_graphs.argtypes = [vp, i32]
     L.vsfeel_bm3d_set_graphs.restype = i32
+    L.vsfeel_bm3d_fail_next_request.argtypes = [vp, i32]
+    L.vsfeel_bm3d_fail_next_request.restype = i32
     L.vsfeel_bm3d_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_int64)]
     L.vsfeel_bm3d_kernel_times.restype = i32

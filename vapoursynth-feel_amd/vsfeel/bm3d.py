@@ -203,6 +203,14 @@ def BM3Dv2(clip: Clip, ref: Clip | None = None, sigma=None, block_step=None, bm_
     return _Bm3dClip(clip, ref, handle, keep)
 
 
+def set_graphs(node: Clip, enable: bool) -> None:
+    """Graph replay of a BM3Dv2 output node's requests on (default) or off."""
+    if not isinstance(node, _Bm3dClip):
+        raise TypeError("not a BM3Dv2 node")
+    if _lib.load().vsfeel_bm3d_set_graphs(node._handle, 1 if enable else 0) != _lib.OK:
+        raise Error("BM3D: set_graphs")
+
+
 def info(node: Clip) -> dict:
     """Derived parameters of a BM3Dv2 output node (scaled sigma, grid, ...)."""
     if not isinstance(node, _Bm3dClip):
