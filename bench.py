@@ -108,19 +108,15 @@ def candidates(w, h, step, bm_range, radius=0, ps_num=0, ps_range=0):
     return float(np.sum(rw) * np.sum(rh)) + float(len(xs) * len(ys) * 2 * radius * ps_num * wh * wh)
 
 
-def mfma_instructions(w, h, step, bm_range, radius=0, ps_num=0, ps_range=0):
-    """16x16x32 MFMA instructions of the precision form's searches per frame:
-    per window 2 per 16x16 tile (h hi, lo) and 6 per 16-row band (the k < 7
-    correction of its first column tile)."""
+def mfma_instructions(w, h, step, bm_range):
+    """16x16x32 MFMA instructions of the precision form's spatial search per
+    frame: per window 2 per 16x16 tile (h hi, lo) and 6 per 16-row band (the
+    k < 7 correction of its first column tile)."""
     xs, ys = ref_positions(w, h, step)
     rw = np.minimum(xs + bm_range, w - 8) - np.maximum(xs - bm_range, 0) + 1
     rh = np.minimum(ys + bm_range, h - 8) - np.maximum(ys - bm_range, 0) + 1
     tr, tc = (rh + 15) // 16, (rw + 15) // 16
-    spatial = float(np.sum(2 * tr[:, None] * tc[None, :] + 6 * tr[:, None] + 0 * tc[None, :]))
-    wh = 2 * ps_range + 1
-    twin = (wh + 15) // 16
-    temporal = float(len(xs) * len(ys) * 2 * radius * ps_num * (2 * twin * twin + 6 * twin))
-    return spatial + temporal
+    return float(np.sum(2 * tr[:, None] * tc[None, :] + 6 * tr[:, None] + 0 * tc[None, :]))
 
 
 def measured_traffic(kernel, workload):
@@ -184,26 +180,28 @@ def roofline(per_kernel, w, h, kw, workload, kernel=None, precision="fp32"):
     tw = 2 * r + 1
     traffic = measured_traffic(dominant, workload)
     if dominant == "bm3d_match" and precision != "fp32":
-        # precision form: the dense 8x8 dot products of the candidates
-        # (norms - 2 dot, 128 FLOP each) against the FP16/BF16 matrix peak
-        ncand = candidates(w, h, kw["block_step"], kw["bm_range"], r, kw.get("ps_num", 0), kw.get("ps_range", 0))
+        # precision form: the spatial search's dense 8x8 dot products (norms
+        # - 2 dot, 128 FLOP per candidate) against the FP16/BF16 matrix peak
+        ncand = candidates(w, h, kw["block_step"], kw["bm_range"])
         work = 128.0 * ncand
-        issued = 16384.0 * mfma_instructions(w, h, kw["block_step"], kw["bm_range"], r, kw.get("ps_num", 0),
-                                             kw.get("ps_range", 0))
+        issued = 16384.0 * mfma_instructions(w, h, kw["block_step"], kw["bm_range"])
         achieved = work / (avg_ms * 1e-3) / 1e12
         return dict(bound="mfma", achieved=achieved, peak=PEAK_MFMA16_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_MFMA16_TFLOPS, traffic=traffic, kernel=dominant,
                     avg_launch_ms=avg_ms, work_per_launch=work,
-                    work_unit=f"FLOP (128 per candidate: the 8x8 dot products of {ncand:.4g} candidates; "
-                              f"{precision.upper()} matrix peak)",
+                    work_unit=f"FLOP (128 per candidate: the 8x8 dot products of the spatial search's "
+                              f"{ncand:.4g} candidates; {precision.upper()} matrix peak)",
                     mfma_flop_issued_per_launch=issued,
                     mfma_issued_tflops=issued / (avg_ms * 1e-3) / 1e12)
-    if dominant == "bm3d_match":
-        work = matching_flops(w, h, kw["block_step"], kw["bm_range"])
-        what = "spatial matching"
-        if r > 0:
-            work += temporal_flops(w, h, kw["block_step"], r, kw["ps_num"], kw["ps_range"])
-            what = "spatial + temporal matching (interior predictor windows)"
+    if dominant in ("bm3d_match", "bm3d_match_temporal"):
+        spatial = matching_flops(w, h, kw["block_step"], kw["bm_range"])
+        temporal = temporal_flops(w, h, kw["block_step"], r, kw["ps_num"], kw["ps_range"]) if r > 0 else 0.0
+        if dominant == "bm3d_match_temporal":
+            work, what = temporal, "temporal predictive matching, interior predictor windows"
+        elif r > 0 and "bm3d_match_temporal" not in per_kernel:
+            work, what = spatial + temporal, "spatial + temporal matching (interior predictor windows)"
+        else:
+            work, what = spatial, "spatial matching"
         achieved = work / (avg_ms * 1e-3) / 1e12
         return dict(bound="valu_fp32", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP32_TFLOPS, traffic=traffic, kernel=dominant,
@@ -311,6 +309,8 @@ def bench_c4(steps, warmup, oracle_threads):
         den = psnr(got, clean0)
         out[prec] = dict(value=nf * steps / secs, unit="frames/s", ms_per_step=secs * 1e3 / steps,
                          kernels_ms_avg={k: v[0] for k, v in per_kernel.items()},
+                         roofline=roofline(per_kernel, w, h, kw, "", kernel="bm3d_match", precision=prec),
+                         roofline_temporal=roofline(per_kernel, w, h, kw, "", kernel="bm3d_match_temporal"),
                          groups_identical_to_plain=float((g == g_plain).all(axis=(2, 3)).mean()),
                          groups_identical_to_fp64=float((g == g64).all(axis=(2, 3)).mean()),
                          max_abs_vs_fp64=float(np.abs(got - ref).max()),
@@ -423,7 +423,8 @@ def main():
         extra["temporal_r2"] = dict(value=plain_r2, unit="frames/s", ms_per_step=s3 * 1e3 / tsteps,
                                     workload=describe(3, c3.shape, kw3, "plain form, luma"),
                                     kernels_ms_avg={k: v[0] for k, v in pk3.items()},
-                                    roofline=roofline(pk3, w3, h3, kw3, wl3, kernel="bm3d_match"))
+                                    roofline=roofline(pk3, w3, h3, kw3, wl3, kernel="bm3d_match_temporal"),
+                                    roofline_spatial=roofline(pk3, w3, h3, kw3, wl3, kernel="bm3d_match"))
     if "mfma" in forms:
         # the precision form on the temporal workload (the north star's
         # 1080p radius 2 target) and on the headline workload (radius 0)

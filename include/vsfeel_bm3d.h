@@ -238,11 +238,12 @@ int vsfeel_bm3d_backward(vsfeel_bm3d *h, const float *grad_out, int32_t grad_pit
  * events are recorded around every launch on the launch stream.
  * vsfeel_bm3d_set_timing resets the counters; vsfeel_bm3d_kernel_times waits
  * for the recorded launches and returns the summed milliseconds and launch
- * counts per kernel (arrays of 8): [0] slot clear, [1] block matching,
- * [2] collaborative filtering + scatter, [3] aggregation, [4] tile reduction,
- * [5] weight box sums of the other frames' slices (radius > 0), [6] sums of
- * the binned other-frame blocks (radius > 0), [7] plane sets of the
- * precision form.
+ * counts per kernel (arrays of 9): [0] slot clear, [1] block matching (at
+ * radius 1-2 its spatial search), [2] collaborative filtering + scatter,
+ * [3] aggregation, [4] tile reduction, [5] weight box sums of the other
+ * frames' slices (radius > 0), [6] sums of the binned other-frame blocks
+ * (radius > 0), [7] plane sets of the precision form, [8] the temporal
+ * predictive search after the spatial one (radius 1-2).
  */
 int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
 int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);

@@ -30,6 +30,12 @@ CASES = {
     # the precision-form workload's parameters (radius 1, sigma 5)
     "c4_small": (lambda s: s.workload(4, frames=3, width=480, height=270),
                  dict(sigma=5.0, block_step=4, bm_range=16, radius=1, ps_num=2, ps_range=7)),
+    # full-size frames: the 1080p temporal workload (radius 2) and the 4K
+    # precision-form workload's own frames (radius 1)
+    "c3_1080p": (lambda s: s.workload(3, frames=5)[0],
+                 dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7, extractor_exp=2)),
+    "c4_4k": (lambda s: s.workload(4, frames=3),
+              dict(sigma=5.0, block_step=4, bm_range=16, radius=1, ps_num=2, ps_range=7)),
     # ragged sizes and a step that leaves runs of unaligned windows
     "ragged": (lambda s: s.gray_clip(10, 2, 77, 45, 5 / 255),
                dict(sigma=4.0, block_step=3, bm_range=5, radius=0)),

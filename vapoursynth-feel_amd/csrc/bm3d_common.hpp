@@ -51,7 +51,6 @@ struct MatchArgs {
     int mfma_run_blocks, mfma_runs_x;
     // nonzero: the temporal search of the precision form runs on MFMA too
     // (bm3d_match_predictive_mfma: radius <= 2, ps_num <= 8)
-    int mfma_temporal;
 };
 
 // Arithmetic of the block-matching distances: the reference's FP32 sums
@@ -85,15 +84,22 @@ __host__ __device__ inline long long plane_set_bytes(int width, int height)
 // LDS of bm3d_match_mfma for a run region of `cols` source columns and a
 // window of `rh` rows: box (rows padded to 16), per wavefront 512 bytes
 // (operand rows, shortlist), h hi + lo (column-major, rows padded to 8) and
-// s (row-major), each 7 rows more than the window's
+// s (row-major, rows of an even number of elements), each 7 rows more than
+// the window's.  The column strides of box and h are odd multiples of 4
+// dwords: the 8 lanes of one pass of a 16-byte LDS read (consecutive
+// columns) then fall on distinct banks.
 constexpr int kMfmaWaveBytes = 512;
+__host__ __device__ inline int mfma_odd4(int dw) { return ((dw >> 2) & 1) ? dw : dw + 4; }
 __host__ __device__ inline int mfma_region_rows(int rh) { return (rh + 7 + 7) & ~7; }
 __host__ __device__ inline int mfma_box_rows(int rh) { return (rh + 15) & ~15; }
+__host__ __device__ inline int mfma_h_stride(int rh) { return 2 * mfma_odd4(mfma_region_rows(rh) / 2); }
+__host__ __device__ inline int mfma_box_stride(int rh) { return mfma_odd4(mfma_box_rows(rh)); }
+__host__ __device__ inline int mfma_s_stride(int cols) { return (cols + 1) & ~1; }
 __host__ __device__ inline long long mfma_rows_lds_bytes(int cols, int rh)
 {
     const long long nhc = cols - 7;
-    return nhc * mfma_box_rows(rh) * 4 + 4LL * kMfmaWaveBytes + 2 * nhc * mfma_region_rows(rh) * 2LL +
-           (rh + 7LL) * cols * 2;
+    return nhc * mfma_box_stride(rh) * 4 + 4LL * kMfmaWaveBytes + 2 * nhc * mfma_h_stride(rh) * 2LL +
+           (rh + 7LL) * mfma_s_stride(cols) * 2;
 }
 
 // Dynamic LDS of bm3d_match with the temporal threshold selection: per

@@ -23,7 +23,11 @@
 
 namespace vsfeel {
 
-hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s);
+// block matching: radius 1-2 with a row-parallel spatial search runs in two
+// passes (the spatial search storing its top-8, then the temporal search:
+// temporal_pass), otherwise in one (temporal_pass false)
+bool match_two_pass(const MatchArgs &a);
+hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s, bool temporal_pass);
 hipError_t launch_estimate(const EstimateArgs &a, int fp_mode, bool final_pass, int mode, hipStream_t s);
 hipError_t launch_divide(const float *num, int num_pitch, const float *den, int den_pitch, float *out, int out_pitch,
                          int width, int height, hipStream_t s);
@@ -77,6 +81,7 @@ enum KernelKind {
     kKernelWeightBox,
     kKernelGather,
     kKernelPlanes,
+    kKernelTemporal,
     kKernelKinds
 };
 
@@ -519,7 +524,6 @@ struct Instance {
             m.pset_elems = (long long)width * m.pset_py;
             m.mfma_run_blocks = mfma_run_blocks;
             m.mfma_runs_x = mfma_runs_x;
-            m.mfma_temporal = radius > 0 && radius <= 2 ? 1 : 0;
         }
         return m;
     }
@@ -539,7 +543,11 @@ struct Instance {
                 }
             }
         }
-        return timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s); });
+        hipError_t e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s, false); });
+        if (e == hipSuccess && match_two_pass(m)) {
+            e = timed_launch(kKernelTemporal, s, [&] { return launch_match(m, fp_mode, s, true); });
+        }
+        return e;
     }
 
     using Saved = SavedState;

@@ -1998,7 +1998,6 @@ __global__ __launch_bounds__(256) void bm3d_tile_tables(int *tab, int width, int
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_match_mfma(const MatchArgs &a, int fp_mode, hipStream_t s);
-hipError_t launch_predictive_mfma(const MatchArgs &a, int fp_mode, hipStream_t s);
 
 // the temporal search after a spatial search that stored its top-8 (radius 1-2)
 static hipError_t launch_temporal(const MatchArgs &a, int fp_mode, hipStream_t s)
@@ -2024,19 +2023,25 @@ static hipError_t launch_temporal(const MatchArgs &a, int fp_mode, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
+bool match_two_pass(const MatchArgs &a)
 {
+    return a.radius > 0 && a.radius < 3 && a.run_blocks > 0;
+}
+
+hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s, bool temporal_pass)
+{
+    if (temporal_pass) {
+        return launch_temporal(a, fp_mode, s);
+    }
     const int nref = a.nx * a.ny;
     const dim3 grid((nref + 3) / 4), block(256);
     const bool exact = a.radius >= 3;
     const bool fused = fp_mode == kFpAll;
     if (!exact && a.operand != kOperandFp32 && a.mfma_run_blocks > 0 && a.run_blocks > 0) {
-        // precision form: the MFMA spatial search, then the temporal search
-        const hipError_t e = launch_match_mfma(a, fp_mode, s);
-        if (e != hipSuccess || a.radius == 0) {
-            return e;
-        }
-        return a.mfma_temporal ? launch_predictive_mfma(a, fp_mode, s) : launch_temporal(a, fp_mode, s);
+        // precision form: the MFMA spatial search; the temporal search after
+        // it keeps exact distances (its predictor windows are small:
+        // gathering MFMA operands for each costs more than the FP32 search)
+        return launch_match_mfma(a, fp_mode, s);
     }
     if (!exact && a.run_blocks > 0) {
         // row-parallel spatial search; at radius 0 it writes the groups
@@ -2052,11 +2057,7 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s)
         } else {
             hipLaunchKernelGGL((bm3d_match_rows<kFpExplicit>), rgrid, block, lds, s, rows);
         }
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess || a.radius == 0) {
-            return e;
-        }
-        return launch_temporal(a, fp_mode, s);
+        return hipGetLastError();
     }
     MatchArgs b = a;
     b.spatial = nullptr;

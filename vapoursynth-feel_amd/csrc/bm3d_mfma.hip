@@ -7,8 +7,12 @@
 //   bm3d_match_mfma             spatial search of the sliding metric (radius
 //                               < 3), a run of ref blocks per workgroup, one
 //                               block per wavefront at a time
-//   bm3d_match_predictive_mfma  temporal predictive search (radius 1-2), one
-//                               ref block per wavefront
+//
+// At radius 1-2 the temporal predictive search that follows keeps the plain
+// form's FP32 distances (bm3d_match_predictive): its windows (ps_num windows
+// of (2 ps_range + 1)^2 candidates per step) are too small for the MFMA
+// operands to be staged once and shared, and gathering them per window from
+// the plane set costs more than the FP32 search.
 //
 // The distance of candidate k of a window row is the reference's sum of 8
 // column SSDs, column c against reference column min(k + c, 7)
@@ -415,11 +419,12 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     const int rx1 = min(x_last + R, W - 8) + 7;
     const int ncols = rx1 - rx0 + 1, nhc = ncols - 7;
     const int CR = mfma_region_rows(rh), CRB = mfma_box_rows(rh), RSR = rh + 7;
+    const int HS = mfma_h_stride(rh), BS = mfma_box_stride(rh), SS = mfma_s_stride(ncols);
     float *boxr = reinterpret_cast<float *>(mlds);
-    unsigned char *wbase = reinterpret_cast<unsigned char *>(boxr + nhc * CRB);
+    unsigned char *wbase = reinterpret_cast<unsigned char *>(boxr + nhc * BS);
     uint16_t *hhi = reinterpret_cast<uint16_t *>(wbase + kMfmaWaves * kMfmaWaveBytes);
-    uint16_t *hlo = hhi + nhc * CR;
-    uint16_t *srr = hlo + nhc * CR;  // [RSR][ncols]
+    uint16_t *hlo = hhi + nhc * HS;
+    uint16_t *srr = hlo + nhc * HS;  // [RSR][SS]
     const int z = A.radius;
     const uint16_t *set = A.pset[z];
     const long long E = A.pset_elems;
@@ -436,21 +441,26 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
             uint32_t vh, vl;
             __builtin_memcpy(&vh, set + o, 4);
             __builtin_memcpy(&vl, set + E + o, 4);
-            lh[i] = vh;
-            ll[i] = vl;
+            lh[c * (HS / 2) + q] = vh;
+            ll[c * (HS / 2) + q] = vl;
         }
-        // s: rows top .. top + RSR - 1, columns rx0 .. rx1
+        // s: rows top .. top + RSR - 1, columns rx0 .. rx1 (the padding
+        // column of an odd count takes the next element of the plane set)
         const uint16_t *gs = set + 2 * E;
-        for (int i = threadIdx.x; i < RSR * ncols; i += 256) {
-            const int r = i / ncols, c = i - r * ncols;
-            srr[i] = gs[(long long)(top + r) * W + rx0 + c];
+        const int sdw = SS / 2;
+        uint32_t *ls = reinterpret_cast<uint32_t *>(srr);
+        for (int i = threadIdx.x; i < RSR * sdw; i += 256) {
+            const int r = i / sdw, q = i - r * sdw;
+            uint32_t v;
+            __builtin_memcpy(&v, gs + (long long)(top + r) * W + rx0 + 2 * q, 4);
+            ls[i] = v;
         }
         // box: region row 16 b + 4 g + r holds window row 16 b + g + 4 r
         const float *gb = reinterpret_cast<const float *>(set + 3 * E);
         for (int i = threadIdx.x; i < nhc * CRB; i += 256) {
             const int c = i / CRB, p = i - c * CRB;
             const int row = (p & ~15) + ((p >> 2) & 3) + 4 * (p & 3);
-            boxr[i] = row < rh ? gb[(long long)(rx0 + c) * py + top + row] : 0.0f;
+            boxr[c * BS + p] = row < rh ? gb[(long long)(rx0 + c) * py + top + row] : 0.0f;
         }
     }
     __syncthreads();
@@ -471,7 +481,7 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
         const int xb = min(bx * s, W - 8);
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         const int rw = right - left + 1;
-        const BlockOperands o = block_operands<OP>(srr + (y - top) * ncols + (xb - rx0), ncols, scratch, lane);
+        const BlockOperands o = block_operands<OP>(srr + (y - top) * SS + (xb - rx0), SS, scratch, lane);
         const uint32_t pm = pos_mask(rw * rh);
         uint32_t m1 = ~0u, m2 = ~0u, m3 = ~0u;
         const uint16_t *s_left = srr + (left - rx0);
@@ -480,7 +490,7 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
             u32x4 a_s[2];
 #pragma unroll
             for (int st = 0; st < 2; ++st) {
-                a_s[st] = load8(s_left + min(m0 + rho + 4 * st + g, RSR - 1) * ncols);
+                a_s[st] = load8(s_left + min(m0 + rho + 4 * st + g, RSR - 1) * SS);
             }
             const int rbase = min(m0 + 8 * g, CR - 8);
             for (int n00 = 0; n00 < rw; n00 += 48) {
@@ -490,9 +500,9 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
                     hc[t] = min(left + n00 + 16 * t + n - rx0, nhc - 1);
-                    bh[t] = *reinterpret_cast<const u32x4 *>(hhi + hc[t] * CR + rbase);
-                    bl[t] = *reinterpret_cast<const u32x4 *>(hlo + hc[t] * CR + rbase);
-                    bx4[t] = *reinterpret_cast<const f32x4 *>(boxr + hc[t] * CRB + m0 + 4 * g);
+                    bh[t] = *reinterpret_cast<const u32x4 *>(hhi + hc[t] * HS + rbase);
+                    bl[t] = *reinterpret_cast<const u32x4 *>(hlo + hc[t] * HS + rbase);
+                    bx4[t] = *reinterpret_cast<const f32x4 *>(boxr + hc[t] * BS + m0 + 4 * g);
                 }
 #pragma unroll
                 for (int t = 0; t < 3; ++t) {
@@ -558,209 +568,6 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     }
 }
 
-// One 16x16 tile of a temporal window (top-left candidate (left, top), tile
-// rows m0.., columns n0..; wave-uniform geometry), operands straight from
-// the frame's plane set in global memory (unaligned 16-byte loads at 32-bit
-// element offsets from the set): the accumulator ends at D~ of the lane's 4
-// candidates.
-template <int OP>
-__device__ __forceinline__ f32x4 window_tile(const BlockOperands &o, const uint16_t *set, uint32_t E, uint32_t py,
-                                            uint32_t W, int left, int top, int m0, int n0, int g, int n, int rho)
-{
-    const uint32_t hx = uint32_t(min(left + n0 + n, int(W) - 8));
-    const uint32_t oh = hx * py + uint32_t(top + m0 + 8 * g);
-    const u32x4 bh = load8(set + oh);
-    const u32x4 bl = load8(set + E + oh);
-    u32x4 a_s[2];
-    if (n0 == 0) {
-#pragma unroll
-        for (int st = 0; st < 2; ++st) {
-            a_s[st] = load8(set + 2 * E + uint32_t(top + m0 + rho + 4 * st + g) * W + uint32_t(left));
-        }
-    }
-    const float *box = reinterpret_cast<const float *>(set + 3 * E);
-    const uint32_t ob = hx * py + uint32_t(top + m0 + g);
-    const float nr = n0 == 0 ? o.nr_first : o.nr7;
-    f32x4 acc;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        acc[r] = box[ob + 4u * r] + nr;
-    }
-    acc = mfma16<OP>(o.a_main, bh, acc);
-    if (n0 == 0) {
-        acc = mfma16<OP>(a_s[0], o.b_d[0], acc);
-        acc = mfma16<OP>(a_s[1], o.b_d[1], acc);
-    }
-    return mfma16<OP>(o.a_main, bl, acc);
-}
-
-// lane l's partner l ^ 32 value of v (v_permlane32_swap)
-__device__ __forceinline__ uint32_t partner32(uint32_t v, int lane)
-{
-    const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-    return lane < 32 ? uint32_t(sw[1]) : uint32_t(sw[0]);
-}
-
-// The SL smallest keys of both directions at once (distinct keys; ~0u never
-// counts): lane l < 32 takes the direction-0 lists (a) of lanes l and
-// l + 32, lane l + 32 the direction-1 lists (b) of both, merged into a
-// sorted 4-entry list; then SL rounds of a half-wave minimum, each half's
-// winner moving on to its next key.  Their seqs go to stage[h][0 .. want[h]).
-__device__ __forceinline__ void extract_both(uint32_t a1, uint32_t a2, uint32_t b1, uint32_t b2, int SL,
-                                             uint32_t pm, uint32_t (*stage)[16], int lane, int (&want)[2])
-{
-    const bool lo = lane < 32;
-    const uint32_t r1 = partner32(lo ? b1 : a1, lane), r2 = partner32(lo ? b2 : a2, lane);
-    const uint32_t o1 = lo ? a1 : b1, o2 = lo ? a2 : b2;
-    // merge of the sorted pairs (o1, o2) and (r1, r2)
-    const uint32_t hi1 = max(o1, r1), lo2 = min(o2, r2);
-    uint32_t m1 = min(o1, r1), m2 = min(hi1, lo2), m3 = max(hi1, lo2), m4 = max(o2, r2);
-    int got = 0;
-    for (int q = 0; q < SL; ++q) {
-        const uint32_t mn = half_min32_u32(m1, lane);
-        const bool valid = mn != ~0u;
-        if (valid && (lane & 31) == 0) {
-            stage[lane >> 5][q] = mn & pm;
-        }
-        got += valid ? 1 : 0;
-        if (m1 == mn) {
-            m1 = m2;
-            m2 = m3;
-            m3 = m4;
-            m4 = ~0u;
-        }
-    }
-    want[0] = __builtin_amdgcn_readlane(got, 0);
-    want[1] = __builtin_amdgcn_readlane(got, 32);
-    wave_lds_sync();
-}
-
-// ---------------------------------------------------------------------------
-// Temporal predictive search of the precision form (radius 1-2,
-// src/bm3d.comp:540-637) after bm3d_match_mfma stored the spatial top-8: one
-// wavefront per ref block.  Each step t tiles the ps_num predictor windows
-// of both directions (frames r - t and r + t; seq base i (2 ps_range + 1)^2
-// for window i), the two directions' windows side by side, into each lane's
-// 2 smallest keys per direction; the 2 ps_num smallest of each direction
-// are rescored exactly and sorted (one 16-lane row per direction); their
-// first ps_num entries are the next step's predictors and the group's
-// insertions, added afterwards in the shader's order like the plain form's.
-// ---------------------------------------------------------------------------
-template <int CM, int OP>
-__global__ __launch_bounds__(64) void bm3d_match_predictive_mfma(MatchArgs A)
-{
-    __shared__ uint16_t scratch[112];
-    __shared__ uint32_t st_seq[2][16];
-    __shared__ unsigned long long st_key[2][16];
-    __shared__ uint32_t st_xy[2][16];
-    const int lane = lane_id();
-    const int blk = xcd_block_id();
-    if (blk >= A.nx * A.ny) {
-        return;
-    }
-    const int by = blk / A.nx;
-    const int bx = blk - by * A.nx;
-    const int W = A.width, H = A.height, r = A.radius;
-    const int x = min(bx * A.step, W - 8);
-    const int y = min(by * A.step, H - 8);
-    const int PR = A.ps_range, PN = A.ps_num;
-    const int wh = 2 * PR + 1, win = wh * wh;
-    const int SL = min(2 * PN, 16);
-    const uint32_t E = uint32_t(A.pset_elems), py = uint32_t(A.pset_py), uW = uint32_t(W);
-    const uint32_t pitch = uint32_t(A.pitch);
-    const uint32_t pm = pos_mask(PN * win);
-    const int g = lane >> 4, n = lane & 15;
-    const int mrow = lane & 15;
-    const int rho = (mrow >> 2) + 4 * (mrow & 3);
-    const BlockOperands o = block_operands<OP>(A.pset[r] + 2 * E + uint32_t(y) * uW + uint32_t(x), W, scratch, lane);
-    const uint32_t ref_off = uint32_t(y) * pitch + uint32_t(x);
-
-    // the spatial top-8 starts the group and predicts (lanes h * 16 + i:
-    // predictor i of direction h)
-    GroupList G;
-    uint32_t pred;
-    {
-        const uint2 e = A.spatial[(long long)blk * 8 + (lane & 7)];
-        G.e = __uint_as_float(e.x);
-        G.xy = e.y;
-        G.z = r;
-        pred = e.y;
-    }
-    uint64_t fkey[2] = {kNoKey, kNoKey};
-    uint32_t fxy[2] = {0u, 0u};
-    for (int t = 1; t <= r; ++t) {
-        const uint16_t *set0 = A.pset[r - t], *set1 = A.pset[r + t];
-        uint32_t a1 = ~0u, a2 = ~0u, b1 = ~0u, b2 = ~0u, unused = ~0u;
-        for (int p = 0; p < PN; ++p) {
-            // window p of both directions (wave-uniform geometry)
-            const uint32_t q0 = uint32_t(__builtin_amdgcn_readfirstlane(__shfl(int(pred), p)));
-            const uint32_t q1 = uint32_t(__builtin_amdgcn_readfirstlane(__shfl(int(pred), 16 + p)));
-            const int c0 = int(q0 & 0xFFFFu), r0 = int(q0 >> 16), c1 = int(q1 & 0xFFFFu), r1 = int(q1 >> 16);
-            const int l0 = max(c0 - PR, 0), t0 = max(r0 - PR, 0), l1 = max(c1 - PR, 0), t1 = max(r1 - PR, 0);
-            const int w0 = min(c0 + PR, W - 8) - l0 + 1, h0 = min(r0 + PR, H - 8) - t0 + 1;
-            const int w1 = min(c1 + PR, W - 8) - l1 + 1, h1 = min(r1 + PR, H - 8) - t1 + 1;
-            const uint32_t sb = uint32_t(p * win);
-            for (int m0 = 0; m0 < max(h0, h1); m0 += 16) {
-                for (int n0 = 0; n0 < max(w0, w1); n0 += 16) {
-                    const f32x4 acc0 = window_tile<OP>(o, set0, E, py, uW, l0, t0, m0, n0, g, n, rho);
-                    const f32x4 acc1 = window_tile<OP>(o, set1, E, py, uW, l1, t1, m0, n0, g, n, rho);
-                    tile_keys<2>(acc0, m0, g, n0 + n, w0, h0, sb, pm, a1, a2, unused);
-                    tile_keys<2>(acc1, m0, g, n0 + n, w1, h1, sb, pm, b1, b2, unused);
-                }
-            }
-        }
-        int want[2];
-        extract_both(a1, a2, b1, b2, SL, pm, st_seq, lane, want);
-        // rescoring, 8 candidates per pass: slot ci = h * SL + q
-        const float inv_win = 1.0f / float(win);
-        for (int p0 = 0; p0 < 2 * SL; p0 += 8) {
-            const int ci = p0 + (lane >> 3);
-            const int h = ci >= SL ? 1 : 0, q = ci - h * SL;
-            const bool on = ci < 2 * SL && q < want[h];
-            const int sq = on ? int(st_seq[h][q]) : 0;
-            const int w = div_small(sq, inv_win), local = sq - w * win;
-            const uint32_t pq = uint32_t(__shfl(int(pred), h * 16 + w));
-            const int pcx = int(pq & 0xFFFFu), pcy = int(pq >> 16);
-            const int left = max(pcx - PR, 0), top = max(pcy - PR, 0);
-            const int rw = min(pcx + PR, W - 8) - left + 1;
-            const int row = div_small(local, 1.0f / float(rw));
-            const int k = local - row * rw;
-            const int cx = left + k, cy = top + row;
-            const float *plane = A.plane[h ? r + t : r - t];
-            const float d = exact_dist8<CM>(A.plane[r], ref_off, plane, uint32_t(cy) * pitch + uint32_t(cx), pitch, k,
-                                            lane & 7);
-            if (on && (lane & 7) == 0) {
-                rescored_key(d, uint32_t(sq), cx, cy, st_key[h][q], st_xy[h][q]);
-            }
-        }
-        wave_lds_sync();
-        // both directions sorted at once: lanes h * 16 + q
-        const int hs = (lane >> 4) & 1, qs = lane & 15;
-        uint64_t key = lane < 32 && qs < want[hs] ? st_key[hs][qs] : kNoKey;
-        uint32_t kxy = lane < 32 && qs < want[hs] ? st_xy[hs][qs] : 0u;
-        wave_lds_sync();
-        if (SL <= 4) {
-            sort16_step<2, 1>(key, kxy, lane);
-            sort16_step<4, 2>(key, kxy, lane);
-            sort16_step<4, 1>(key, kxy, lane);
-        } else {
-            sort16_dpp(key, kxy, lane);
-        }
-        if (!(key < kPlaceholderKey)) {
-            key = kPlaceholderKey;  // fewer candidates than the step keeps
-            kxy = 0u;
-        }
-        pred = kxy;
-        // lanes h * 32 + i: step t's entry i of direction h
-        const int src = (lane >> 5) * 16 + (lane & 15);
-        fkey[t - 1] = (uint64_t(uint32_t(__shfl(int(uint32_t(key >> 32)), src))) << 32) |
-                      uint32_t(__shfl(int(uint32_t(key)), src));
-        fxy[t - 1] = uint32_t(__shfl(int(kxy), src));
-    }
-    insert_steps(G, fkey, fxy, r, PN, lane);
-    finish_group(A, G, blk, x, y, lane);
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -805,27 +612,6 @@ hipError_t launch_match_mfma(const MatchArgs &a, int fp_mode, hipStream_t s)
         }
     }
 #undef VSFEEL_MM
-    return hipGetLastError();
-}
-
-hipError_t launch_predictive_mfma(const MatchArgs &a, int fp_mode, hipStream_t s)
-{
-    const dim3 grid(a.nx * a.ny), block(64);
-#define VSFEEL_PM(CM, OP) hipLaunchKernelGGL((bm3d_match_predictive_mfma<CM, OP>), grid, block, 0, s, a)
-    if (fp_mode == kFpAll) {
-        if (a.operand == kOperandBf16) {
-            VSFEEL_PM(kFpAll, kOperandBf16);
-        } else {
-            VSFEEL_PM(kFpAll, kOperandFp16);
-        }
-    } else {
-        if (a.operand == kOperandBf16) {
-            VSFEEL_PM(kFpExplicit, kOperandBf16);
-        } else {
-            VSFEEL_PM(kFpExplicit, kOperandFp16);
-        }
-    }
-#undef VSFEEL_PM
     return hipGetLastError();
 }
 

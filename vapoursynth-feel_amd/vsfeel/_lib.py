@@ -37,7 +37,7 @@ EXPORTS = (
 )
 
 KERNEL_NAMES = ("fill_zero", "bm3d_match", "bm3d_estimate", "bm3d_aggregate", "bm3d_tile_reduce",
-                "bm3d_weight_box", "bm3d_gather", "bm3d_planes")
+                "bm3d_weight_box", "bm3d_gather", "bm3d_planes", "bm3d_match_temporal")
 
 
 class Format(ctypes.Structure):
