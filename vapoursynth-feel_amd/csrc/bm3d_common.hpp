@@ -41,16 +41,15 @@ struct MatchArgs {
     // nonzero: the temporal search runs both directions in one wavefront
     // (bm3d_match_predictive; needs ps_num * (2 ps_range + 1) <= 32)
     int pred_pair;
-    // precision form (operand != kOperandFp32): MFMA distances from the
-    // window frames' plane sets (bm3d_planes, layout below), spatial runs of
-    // mfma_run_blocks ref blocks per workgroup (mfma_runs_x per grid row)
+    // precision form (operand != kOperandFp32): MFMA distances of the
+    // spatial search from the reference frame's plane set (bm3d_planes,
+    // layout below), runs of mfma_run_blocks ref blocks per workgroup
+    // (mfma_runs_x per grid row)
     int operand;
-    const uint16_t *pset[kMaxWindow];
+    const uint16_t *pset;
     int pset_py;
     long long pset_elems;
     int mfma_run_blocks, mfma_runs_x;
-    // nonzero: the temporal search of the precision form runs on MFMA too
-    // (bm3d_match_predictive_mfma: radius <= 2, ps_num <= 8)
 };
 
 // Arithmetic of the block-matching distances: the reference's FP32 sums
@@ -114,6 +113,9 @@ __host__ __device__ inline long long temporal_lds_bytes(int t_rows, int ps_range
 // survivor keys and positions, the reference block, the 2 ps_num predictor
 // windows of a step ((2 ps_range + 8)^2 floats each, column-major) and a
 // distance row per lane
+// rows a lane of bm3d_match_predictive loads per batch
+constexpr int kPredBatch = 24;
+
 __host__ __device__ inline long long predictive_lds_bytes(int ps_num, int ps_range)
 {
     const long long wh = 2LL * ps_range + 1, wc = wh + 7;

@@ -143,7 +143,7 @@ struct Instance {
     // distances of the block matching (OperandType): FP32 (plain form) or
     // MFMA FP16 / BF16 (precision form, with a plane set per window position)
     int operand = kOperandFp32;
-    uint16_t *psets = nullptr;  // tw plane sets of pset_bytes
+    uint16_t *psets = nullptr;  // the reference frame's plane set (precision form)
     long long pset_bytes = 0;
     int mfma_run_blocks = 0, mfma_runs_x = 0;
     int tw = 1, src_ring = 1, res_cap = 1, clips = 1;
@@ -517,9 +517,7 @@ struct Instance {
         m.pred_pair = pred_pair;
         m.operand = operand;
         if (operand != kOperandFp32) {
-            for (int z = 0; z < tw; ++z) {
-                m.pset[z] = reinterpret_cast<const uint16_t *>(reinterpret_cast<const char *>(psets) + z * pset_bytes);
-            }
+            m.pset = psets;
             m.pset_py = plane_py(height);
             m.pset_elems = (long long)width * m.pset_py;
             m.mfma_run_blocks = mfma_run_blocks;
@@ -529,18 +527,16 @@ struct Instance {
     }
 
     // block matching of one estimate: in the precision form first the plane
-    // sets of the window frames, then the search
+    // set of the reference frame (the spatial search's operands), then the
+    // search
     hipError_t match(const MatchArgs &m, const float *const *search, int in_pitch, hipStream_t s)
     {
         if (operand != kOperandFp32) {
-            for (int z = 0; z < tw; ++z) {
-                uint16_t *set = const_cast<uint16_t *>(m.pset[z]);
-                const hipError_t e = timed_launch(kKernelPlanes, s, [&] {
-                    return launch_planes(search[z], in_pitch, width, height, set, operand, s);
-                });
-                if (e != hipSuccess) {
-                    return e;
-                }
+            const hipError_t e = timed_launch(kKernelPlanes, s, [&] {
+                return launch_planes(search[radius], in_pitch, width, height, psets, operand, s);
+            });
+            if (e != hipSuccess) {
+                return e;
             }
         }
         hipError_t e = timed_launch(kKernelMatch, s, [&] { return launch_match(m, fp_mode, s, false); });
@@ -1083,7 +1079,7 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
     }
     if (d.operand != kOperandFp32) {
         d.pset_bytes = (plane_set_bytes(d.width, d.height) + 255) & ~255LL;
-        if ((e = hipMalloc(&d.psets, size_t(d.pset_bytes) * d.tw)) != hipSuccess) {
+        if ((e = hipMalloc(&d.psets, size_t(d.pset_bytes))) != hipSuccess) {
             return dev_fail(e, "hipMalloc");
         }
     }

@@ -793,8 +793,6 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
     return L;
 }
 
-constexpr int kPredBatch = 24;  // window rows a lane loads in one batch
-
 template <int CM>
 __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
 {
@@ -849,36 +847,55 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         }
         // stage the windows: lane 32*j + c loads column c of window
         // (h, p0 + j) of both directions h, every row, before storing any;
-        // rows and columns past a window repeat its last one (never read)
+        // columns past a window repeat its last one
         for (int p0 = 0; p0 < PN; p0 += 2) {
             const int p = p0 + half;
             const bool on = p < PN && sl < wc;
             float v[2][kPredBatch];
-            // byte offsets in the frame (a plane is below 4 GiB): the lane's
-            // column in the window's first row and in its last row; a row's
-            // offset steps by pitch4 and is clamped to the last row's
-            uint32_t base[2], lastrow[2];
+            // byte offset of the lane's column in the window's first row (a
+            // plane is below 2 GiB); rows past the window are loaded too but
+            // never read, and buffer loads return 0 past the frame's last
+            // row instead of reading beyond the plane
+            uint32_t base[2];
             const uint32_t pitch4 = uint32_t(pitch) * 4u;
+            bool inside = true;  // every batch row lies in the frame
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const uint32_t q = uint32_t(__shfl(int(pred), h * 32 + min(p, PN - 1)));
                 const int cx = int(q & 0xFFFFu), cy = int(q >> 16);
                 const int left = max(cx - PR, 0), top = max(cy - PR, 0);
                 const int ncol = min(cx + PR, W - 8) - left + 8;
-                const int nrow = min(cy + PR, H - 8) - top + 8;
                 base[h] = uint32_t(top * pitch + left + min(sl, ncol - 1)) * 4u;
-                lastrow[h] = base[h] + uint32_t(nrow - 1) * pitch4;
+                inside = inside && top + (wc + kPredBatch - 1) / kPredBatch * kPredBatch <= H;
             }
+            // usually every window's batches lie in the frame: the row's
+            // offset (wave-uniform) goes in the scalar offset, outside the
+            // loads' range check, so no per-row address arithmetic
+            const bool fast = __ballot(on && !inside) == 0;
             for (int i0 = 0; i0 < wc; i0 += kPredBatch) {
                 if (on) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        const char *plane = reinterpret_cast<const char *>(A.plane[h ? r + t : r - t]);
-                        uint32_t off = base[h] + uint32_t(i0) * pitch4;
+                        const __amdgpu_buffer_rsrc_t rows = __builtin_amdgcn_make_buffer_rsrc(
+                            const_cast<float *>(A.plane[h ? r + t : r - t]), 0, H * pitch * 4, 0x00020000);
+                        if (fast) {
 #pragma unroll
-                        for (int u = 0; u < kPredBatch; ++u) {
-                            v[h][u] = *reinterpret_cast<const float *>(plane + min(off, lastrow[h]));
-                            off += pitch4;
+                            for (int u = 0; u < kPredBatch; ++u) {
+                                v[h][u] = __builtin_bit_cast(
+                                    float, __builtin_amdgcn_raw_buffer_load_b32(rows, int(base[h]),
+                                                                               int(uint32_t(i0 + u) * pitch4), 0));
+                            }
+                        } else {
+                            // (an empty asm keeps the per-row offsets in
+                            // this branch instead of hoisted before it)
+                            uint32_t b = base[h];
+                            asm volatile("" : "+v"(b));
+#pragma unroll
+                            for (int u = 0; u < kPredBatch; ++u) {
+                                v[h][u] = __builtin_bit_cast(
+                                    float, __builtin_amdgcn_raw_buffer_load_b32(
+                                               rows, int(b + uint32_t(i0 + u) * pitch4), 0, 0));
+                            }
                         }
                     }
 #pragma unroll
@@ -905,7 +922,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         float *drow = tdist + lane * wh;
         uint3 m = make_uint3(~0u, ~0u, ~0u);
         if (row_ok) {
-            m = scan_row_sliding<CM, 4>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
+            m = scan_row_sliding<CM, 5>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
         }
         wave_lds_sync();
         // only the step's first ps_num (distance, seq) keys are used (the

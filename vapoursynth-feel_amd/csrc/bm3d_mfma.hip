@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     uint16_t *hlo = hhi + nhc * HS;
     uint16_t *srr = hlo + nhc * HS;  // [RSR][SS]
     const int z = A.radius;
-    const uint16_t *set = A.pset[z];
+    const uint16_t *set = A.pset;
     const long long E = A.pset_elems;
     const int py = A.pset_py;
     {
