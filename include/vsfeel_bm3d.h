@@ -249,18 +249,21 @@ int vsfeel_bm3d_set_timing(vsfeel_bm3d *h, int32_t enable);
 int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);
 
 /*
- * Graph replay (on by default).  vsfeel_bm3d_process_device captures its
- * launch sequence (per output frame: slot clears, matching, filtering, the
- * other frames' sums and weights at radius > 0, tile reduction,
- * aggregation) into a HIP graph on the instance's stream, and later calls
- * with identical arguments replay it, ordered after and before the caller's
- * stream by events.  vsfeel_bm3d_get_frame launches a request's kernels as
- * one graph per output slot and recompute pattern (the number of estimates
- * the request runs), captured on an idle stream and updated in place
- * (hipGraphExecUpdate) with the request's ring slots and frame.  The
- * counterpart of the reference's per-frame command-buffer recording
- * (src/bm3d.cpp:486-623, :899-967).  Timing (vsfeel_bm3d_set_timing)
- * launches directly instead.
+ * Graph launches; `enable` is a bit set (default 1).  Bit 0:
+ * vsfeel_bm3d_process_device captures its launch sequence (per output frame:
+ * slot clears, matching, filtering, the other frames' sums and weights at
+ * radius > 0, tile reduction, aggregation) into a HIP graph on the
+ * instance's stream, and later calls with identical arguments replay it,
+ * ordered after and before the caller's stream by events.  Bit 1:
+ * vsfeel_bm3d_get_frame launches a request's kernels as one graph per
+ * output slot and recompute pattern (the number of estimates the request
+ * runs), captured on an idle stream and updated in place
+ * (hipGraphExecUpdate) with the request's ring slots and frame; off by
+ * default, as capture and update under the handle's mutex cost more host
+ * time than the request's direct launches.  The counterpart of the
+ * reference's per-frame command-buffer recording (src/bm3d.cpp:486-623,
+ * :899-967).  Timing (vsfeel_bm3d_set_timing) launches directly instead.
+ * Values outside 0-3: VSFEEL_BM3D_EINVAL.
  */
 int vsfeel_bm3d_set_graphs(vsfeel_bm3d *h, int32_t enable);
 

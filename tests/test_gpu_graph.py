@@ -5,9 +5,10 @@ with a given set of arguments and replays it afterwards.  Every replay, with
 the program's own eager work and synchronisations between replays, must give
 bit-identical frames to the directly launched sequence (the accumulation is
 integer fixed point, so the results are deterministic), also after the clip
-memory is rewritten in place and after the output moves.  The host-buffer
-requests (getFrame) launch from per-slot graphs updated per request, and must
-equal direct launches in any order and from several threads.
+memory is rewritten in place and after the output moves.  Host-buffer
+requests (getFrame) with request graphs on launch from per-slot graphs
+updated per request, and must equal direct launches in any order and from
+several threads.
 """
 
 from __future__ import annotations
@@ -94,6 +95,8 @@ def test_host_requests_from_graphs_match_direct_launches(radius, extractor_exp):
     want = [direct.get_frame(n)[0] for n in range(9)]
 
     node = BM3Dv2(Clip.gray(clip), **kw)
+    assert L.vsfeel_bm3d_set_graphs(node._handle, 4) == _lib.EINVAL
+    assert L.vsfeel_bm3d_set_graphs(node._handle, 3) == _lib.OK  # request graphs on (off by default)
     for n in [4, 0, 8, 5, 5, 1, 7]:  # seeks (whole windows), repeats, neighbours
         assert np.array_equal(node.get_frame(n)[0], want[n]), f"graph frame {n} differs"
     got, errors = {}, []
@@ -116,6 +119,6 @@ def test_host_requests_from_graphs_match_direct_launches(radius, extractor_exp):
         assert len(frames) == 4
         for f in frames:
             assert np.array_equal(f, want[n]), f"graph frame {n} differs under parallel requests"
-    # switching graphs off on a live node drops the slots' graphs
-    assert L.vsfeel_bm3d_set_graphs(node._handle, 0) == _lib.OK
+    # switching request graphs off on a live node drops the slots' graphs
+    assert L.vsfeel_bm3d_set_graphs(node._handle, 1) == _lib.OK
     assert np.array_equal(node.get_frame(3)[0], want[3])

@@ -26,7 +26,7 @@ def test_defaults_are_the_reference_harness_defaults(capsys):
         pipe_bench.run = orig
     # BM3Dv2 defaults of the reference harness (benchmark/bench.py:167-180)
     assert seen == dict(frames=1000, synthetic=True, width=1920, height=1080, sigma=0.7, radius=2,
-                        bm_range=16, ps_range=7, block_step=4, num_streams=4, graphs=True)
+                        bm_range=16, ps_range=7, block_step=4, num_streams=4, graphs=False)
     assert capsys.readouterr().out.strip() == "Output 1000 frames in 20.00 seconds (50.00 fps)"
 
 

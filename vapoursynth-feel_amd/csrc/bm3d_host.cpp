@@ -203,7 +203,8 @@ struct Instance {
                    count == o.count && out_pitch == o.out_pitch && out == o.out;
         }
     };
-    bool graphs = true;
+    bool graphs = true;            // process_device replays its graph
+    bool request_graphs = false;   // get_frame launches from per-slot graphs
     hipGraphExec_t graph_exec = nullptr;
     GraphKey graph_key{};
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;  // orders the instance stream with a null-stream caller
@@ -1395,7 +1396,7 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
         VSFEEL_HIP(d.end_on(s), "hipEventRecord");
         return fail(err, err_len, VSFEEL_BM3D_EDEVICE, "launch for frame " + std::to_string(n) + " failed (test hook)");
     }
-    if (d.graphs && !d.timing) {
+    if (d.request_graphs && !d.timing) {
         // the kernels from the slot's graph for this recompute pattern: captured
         // on the idle capture stream, then updated into the slot's executable
         // graph (whose last launch has finished: the slot's previous request
@@ -1881,22 +1882,25 @@ int vsfeel_bm3d_set_graphs(vsfeel_bm3d *h, int32_t enable)
     if (!h) {
         return VSFEEL_BM3D_EINVAL;
     }
+    if (enable < 0 || enable > 3) {
+        return VSFEEL_BM3D_EINVAL;
+    }
     Instance &d = h->inst;
     std::lock_guard<std::mutex> lock(d.mu);
-    d.graphs = enable != 0;
-    if (!d.graphs) {
-        if (d.device >= 0) {
-            (void)hipSetDevice(d.device);
-        }
-        if (d.stream) {
-            (void)hipStreamSynchronize(d.stream);
-        }
-        if (d.graph_exec) {
-            (void)hipGraphExecDestroy(d.graph_exec);
-            d.graph_exec = nullptr;
-        }
-        // requests that hold a slot launched their graph before it was
-        // synchronized above
+    d.graphs = (enable & 1) != 0;
+    d.request_graphs = (enable & 2) != 0;
+    if (d.device >= 0) {
+        (void)hipSetDevice(d.device);
+    }
+    if (d.stream && ((!d.graphs && d.graph_exec) || !d.request_graphs)) {
+        // (requests that hold a slot launched its graph before this)
+        (void)hipStreamSynchronize(d.stream);
+    }
+    if (!d.graphs && d.graph_exec) {
+        (void)hipGraphExecDestroy(d.graph_exec);
+        d.graph_exec = nullptr;
+    }
+    if (!d.request_graphs) {
         for (Instance::OutSlot &q : d.outs) {
             d.drop_slot_graphs(q);
         }

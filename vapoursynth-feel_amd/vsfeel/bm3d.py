@@ -203,11 +203,13 @@ def BM3Dv2(clip: Clip, ref: Clip | None = None, sigma=None, block_step=None, bm_
     return _Bm3dClip(clip, ref, handle, keep)
 
 
-def set_graphs(node: Clip, enable: bool) -> None:
-    """Graph replay of a BM3Dv2 output node's requests on (default) or off."""
+def set_graphs(node: Clip, requests: bool, device: bool = True) -> None:
+    """Graph launches of a BM3Dv2 output node: per-request graphs of its frame
+    requests (off by default) and graph replay of device batches (on)."""
     if not isinstance(node, _Bm3dClip):
         raise TypeError("not a BM3Dv2 node")
-    if _lib.load().vsfeel_bm3d_set_graphs(node._handle, 1 if enable else 0) != _lib.OK:
+    flags = (1 if device else 0) | (2 if requests else 0)
+    if _lib.load().vsfeel_bm3d_set_graphs(node._handle, flags) != _lib.OK:
         raise Error("BM3D: set_graphs")
 
 
