@@ -116,10 +116,19 @@ __host__ __device__ inline long long temporal_lds_bytes(int t_rows, int ps_range
 // rows a lane of bm3d_match_predictive loads per batch
 constexpr int kPredBatch = 24;
 
+// window stride (floats) of bm3d_match_predictive's staged windows: 16 banks
+// modulo 32 past the window's wc x wc floats, so that the row scan's two
+// windows of a half wave (a window row per lane) fall on disjoint banks
+__host__ __device__ inline int predictive_win_stride(int ps_range)
+{
+    const int wc = 2 * ps_range + 8;
+    return wc * wc + ((48 - (wc * wc) % 32) % 32);
+}
+
 __host__ __device__ inline long long predictive_lds_bytes(int ps_num, int ps_range)
 {
-    const long long wh = 2LL * ps_range + 1, wc = wh + 7;
-    return 64LL * 8 + 64LL * 4 + 64LL * 4 + 4LL * 2 * ps_num * wc * wc + 4LL * 64 * wh;
+    const long long wh = 2LL * ps_range + 1;
+    return 64LL * 8 + 64LL * 4 + 64LL * 4 + 4LL * 2 * ps_num * predictive_win_stride(ps_range) + 4LL * 64 * wh;
 }
 
 // LDS bytes of bm3d_match_rows: survivor stages of its 4 wavefronts (64 keys

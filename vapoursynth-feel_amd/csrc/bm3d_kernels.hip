@@ -813,8 +813,9 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
     unsigned long long *sk = plds;                                // [2][32] survivor keys
     uint32_t *sxy = reinterpret_cast<uint32_t *>(plds + 64);      // [2][32] their positions
     float *cur = reinterpret_cast<float *>(sxy + 64);             // cur[c*8+i] = pixel (x+c, y+i)
-    float *regions = cur + 64;                                    // [2 PN][wc columns][wc rows]
-    float *tdist = regions + 2 * PN * wc * wc;                    // [64 lanes][wh]
+    const int WS = predictive_win_stride(PR);
+    float *regions = cur + 64;                                    // [2 PN][wc columns][wc rows], stride WS
+    float *tdist = regions + 2 * PN * WS;                         // [64 lanes][wh]
 
     cur[lane] = A.plane[r][(long long)(y + (lane & 7)) * pitch + x + (lane >> 3)];
 
@@ -900,7 +901,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
                     }
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        float *dst = regions + ((h * PN + p) * wc + sl) * wc;
+                        float *dst = regions + (h * PN + p) * WS + sl * wc;
 #pragma unroll
                         for (int u = 0; u < kPredBatch; ++u) {
                             if (i0 + u < wc) {
@@ -922,7 +923,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         float *drow = tdist + lane * wh;
         uint3 m = make_uint3(~0u, ~0u, ~0u);
         if (row_ok) {
-            m = scan_row_sliding<CM, 5>(cur, 8, regions + (half * PN + w) * wc * wc + rr, wc, rw, drow, pm);
+            m = scan_row_sliding<CM, 5>(cur, 8, regions + (half * PN + w) * WS + rr, wc, rw, drow, pm);
         }
         wave_lds_sync();
         // only the step's first ps_num (distance, seq) keys are used (the
