@@ -430,37 +430,72 @@ __global__ __launch_bounds__(256) void bm3d_match_mfma(MatchArgs A)
     const long long E = A.pset_elems;
     const int py = A.pset_py;
     {
+        // (the loops step their 2-D index by the block size without a
+        // division per element: (major, minor) += (256 / n, 256 % n))
+        const uint32_t uE = uint32_t(E), upy = uint32_t(py);
         // h (hi, lo): a column's CR rows from the window's top row, as
         // dwords (unaligned loads)
-        const int dw = CR / 2;
-        uint32_t *lh = reinterpret_cast<uint32_t *>(hhi);
-        uint32_t *ll = reinterpret_cast<uint32_t *>(hlo);
-        for (int i = threadIdx.x; i < nhc * dw; i += 256) {
-            const int c = i / dw, q = i - c * dw;
-            const long long o = (long long)(rx0 + c) * py + top + 2 * q;
-            uint32_t vh, vl;
-            __builtin_memcpy(&vh, set + o, 4);
-            __builtin_memcpy(&vl, set + E + o, 4);
-            lh[c * (HS / 2) + q] = vh;
-            ll[c * (HS / 2) + q] = vl;
+        {
+            const int dw = CR / 2, sc = 256 / dw, sq = 256 - sc * dw;
+            uint32_t *lh = reinterpret_cast<uint32_t *>(hhi);
+            uint32_t *ll = reinterpret_cast<uint32_t *>(hlo);
+            int c = int(threadIdx.x) / dw, q = int(threadIdx.x) - c * dw;
+            while (c < nhc) {
+                const uint32_t o = uint32_t(rx0 + c) * upy + uint32_t(top + 2 * q);
+                uint32_t vh, vl;
+                __builtin_memcpy(&vh, set + o, 4);
+                __builtin_memcpy(&vl, set + uE + o, 4);
+                lh[c * (HS / 2) + q] = vh;
+                ll[c * (HS / 2) + q] = vl;
+                q += sq;
+                c += sc;
+                if (q >= dw) {
+                    q -= dw;
+                    ++c;
+                }
+            }
         }
         // s: rows top .. top + RSR - 1, columns rx0 .. rx1 (the padding
         // column of an odd count takes the next element of the plane set)
-        const uint16_t *gs = set + 2 * E;
-        const int sdw = SS / 2;
-        uint32_t *ls = reinterpret_cast<uint32_t *>(srr);
-        for (int i = threadIdx.x; i < RSR * sdw; i += 256) {
-            const int r = i / sdw, q = i - r * sdw;
-            uint32_t v;
-            __builtin_memcpy(&v, gs + (long long)(top + r) * W + rx0 + 2 * q, 4);
-            ls[i] = v;
+        {
+            const uint16_t *gs = set + 2 * uE;
+            const int sdw = SS / 2, sr = 256 / sdw, sq = 256 - sr * sdw;
+            uint32_t *ls = reinterpret_cast<uint32_t *>(srr);
+            int r = int(threadIdx.x) / sdw, q = int(threadIdx.x) - r * sdw;
+            while (r < RSR) {
+                uint32_t v;
+                __builtin_memcpy(&v, gs + uint32_t(top + r) * uint32_t(W) + uint32_t(rx0 + 2 * q), 4);
+                ls[r * sdw + q] = v;
+                q += sq;
+                r += sr;
+                if (q >= sdw) {
+                    q -= sdw;
+                    ++r;
+                }
+            }
         }
-        // box: region row 16 b + 4 g + r holds window row 16 b + g + 4 r
-        const float *gb = reinterpret_cast<const float *>(set + 3 * E);
-        for (int i = threadIdx.x; i < nhc * CRB; i += 256) {
-            const int c = i / CRB, p = i - c * CRB;
-            const int row = (p & ~15) + ((p >> 2) & 3) + 4 * (p & 3);
-            boxr[c * BS + p] = row < rh ? gb[(long long)(rx0 + c) * py + top + row] : 0.0f;
+        // box: region rows 16 b + 4 g .. + 3 hold window rows 16 b + g + 4 r
+        // (r = 0..3), 16 bytes per thread and step
+        {
+            const float *gb = reinterpret_cast<const float *>(set + 3 * uE);
+            const int nu = CRB / 4, sc = 256 / nu, su = 256 - sc * nu;
+            int c = int(threadIdx.x) / nu, u = int(threadIdx.x) - c * nu;
+            while (c < nhc) {
+                const int row0 = 16 * (u >> 2) + (u & 3);
+                const uint32_t col = uint32_t(rx0 + c) * upy + uint32_t(top + row0);
+                f32x4 v;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    v[r] = row0 + 4 * r < rh ? gb[col + 4u * r] : 0.0f;
+                }
+                *reinterpret_cast<f32x4 *>(boxr + c * BS + 4 * u) = v;
+                u += su;
+                c += sc;
+                if (u >= nu) {
+                    u -= nu;
+                    ++c;
+                }
+            }
         }
     }
     __syncthreads();
