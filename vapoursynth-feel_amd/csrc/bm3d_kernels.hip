@@ -636,7 +636,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const int left = max(xb - R, 0), right = min(xb + R, W - 8);
         // the reference block lies in the region: column c at cb + c * CS
         const float *cb = region + (xb - rx0) * CS + (y - top);
-        const uint3 t3 = scan_row_sliding<CM, 8>(cb, CS, region + (left - rx0) * CS + r, CS, right - left + 1,
+        const uint3 t3 = scan_row_sliding<CM, 11>(cb, CS, region + (left - rx0) * CS + r, CS, right - left + 1,
                                               dist + (b * rh + r) * A.dist_cols, pm);
         rowtop[3 * (b * rh + r)] = t3.x;
         rowtop[3 * (b * rh + r) + 1] = t3.y;
