@@ -259,8 +259,9 @@ int vsfeel_bm3d_kernel_times(vsfeel_bm3d *h, double *ms, int64_t *count);
  * output slot and recompute pattern (the number of estimates the request
  * runs), captured on an idle stream and updated in place
  * (hipGraphExecUpdate) with the request's ring slots and frame; off by
- * default, as capture and update under the handle's mutex cost more host
- * time than the request's direct launches.  The counterpart of the
+ * default: measured no faster than the request's direct launches, whose
+ * host cost is small next to its copies, while capture and update run
+ * under the handle's mutex.  The counterpart of the
  * reference's per-frame command-buffer recording (src/bm3d.cpp:486-623,
  * :899-967).  Timing (vsfeel_bm3d_set_timing) launches directly instead.
  * Values outside 0-3: VSFEEL_BM3D_EINVAL.
