@@ -133,3 +133,47 @@ def test_precision_arguments_rejected():
         Instance(5, 64, 64, sigma=3.0, radius=3, precision="fp16")
     with pytest.raises(Error, match="precision"):
         Instance(2, 64, 64, sigma=3.0, precision=3)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_final_pass_groups_agree_with_plain(precision):
+    # the Wiener pass matches on the ref clip: the plane set is the ref
+    # frame's (src/bm3d.comp:401-443 with the groups of src/bm3d.comp:451-663)
+    torch = _torch()
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+
+    clip = synth.workload(4, frames=4, width=320, height=192)
+    nf, h, w = clip.shape
+    kw = dict(sigma=5.0, block_step=4, bm_range=12, radius=1, ps_num=2, ps_range=5)
+    print(f"final pass {precision} clip={clip.shape}", flush=True)
+    x = torch.from_numpy(clip).to("cuda:0")
+    basic = Instance(nf, h, w, **kw).process(x)
+    plain = Instance(nf, h, w, final=True, **kw)
+    fast = Instance(nf, h, w, final=True, precision=precision, **kw)
+    gp = np.stack([plain.groups(x, m, ref=basic)[0] for m in range(nf)])
+    gf = np.stack([fast.groups(x, m, ref=basic)[0] for m in range(nf)])
+    same = (gp == gf).all(axis=(3, 4))
+    share = float(same.mean())
+    print(f"  identical groups {share:.6f} ({int((~same).sum())} of {same.size} differ)", flush=True)
+    assert share >= AGREE
+    op = plain.process(x, ref=basic).cpu().numpy()
+    of = fast.process(x, ref=basic).cpu().numpy()
+    assert np.abs(op - of).max() <= (1e-5 if share == 1.0 else 0.05)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_host_requests_equal_the_device_path(precision):
+    # BM3Dv2(..., precision=...) through the host-buffer path (getFrame)
+    torch = _torch()
+    from vsfeel import BM3Dv2, Clip, synth
+    from vsfeel.torch_bm3d import Instance
+
+    clip = synth.workload(3, frames=6, width=192, height=128)[0]
+    nf, h, w = clip.shape
+    kw = dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7, extractor_exp=2)
+    print(f"host path {precision} clip={clip.shape}", flush=True)
+    want = Instance(nf, h, w, precision=precision, **kw).process(torch.from_numpy(clip).to("cuda:0")).cpu().numpy()
+    node = BM3Dv2(Clip.gray(clip), precision=precision, **kw)
+    for n in (3, 0, 5, 1):
+        assert np.array_equal(node.get_frame(n)[0], want[n]), f"frame {n} differs"
