@@ -36,6 +36,13 @@ CASES = {
                  dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7, extractor_exp=2)),
     "c4_4k": (lambda s: s.workload(4, frames=3),
               dict(sigma=5.0, block_step=4, bm_range=16, radius=1, ps_num=2, ps_range=7)),
+    # the CPU-runnable workload's parameters (step 8, range 9: 19-wide windows)
+    "c1": (lambda s: s.workload(1, frames=2),
+           dict(sigma=3.0, block_step=8, bm_range=9, radius=0)),
+    # windows wider than 48 columns: a second group of column tiles without
+    # the k < 7 correction
+    "wide_range": (lambda s: s.gray_clip(12, 2, 200, 120, 4 / 255),
+                   dict(sigma=3.0, block_step=4, bm_range=28, radius=0)),
     # ragged sizes and a step that leaves runs of unaligned windows
     "ragged": (lambda s: s.gray_clip(10, 2, 77, 45, 5 / 255),
                dict(sigma=4.0, block_step=3, bm_range=5, radius=0)),
