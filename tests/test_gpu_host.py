@@ -205,3 +205,32 @@ def test_failed_request_leaves_the_handle_usable(stage):
         for f in frames:
             assert np.array_equal(f, want[n]), f"frame {n} differs next to a failed request"
     assert np.array_equal(node.get_frame(n_bad)[0], want[n_bad])
+
+
+def test_missing_plane_leaves_the_ring_intact():
+    # a request whose window lacks a plane it must upload fails before it
+    # enqueues any upload ("missing plane of frame f"); the node then serves
+    # every frame as before
+    import ctypes
+    _torch()
+    from vsfeel import BM3Dv2, Clip, _lib, synth
+    clip = synth.workload(3, frames=6, width=128, height=80)[0]
+    kw = dict(sigma=3.0, block_step=4, bm_range=8, radius=1, ps_num=2, ps_range=4, extractor_exp=2)
+    print(f"missing plane clip={clip.shape}", flush=True)
+    want = _device_frames(clip, kw)
+    node = BM3Dv2(Clip.gray(clip), **kw)
+    L = _lib.load()
+    planes = [np.ascontiguousarray(clip[f]) for f in range(6)]
+    first, last = ctypes.c_int32(), ctypes.c_int32()
+    assert L.vsfeel_bm3d_frame_window(node._handle, 2, ctypes.byref(first), ctypes.byref(last)) == _lib.OK
+    fs = list(range(first.value, last.value + 1))
+    VP = ctypes.c_void_p
+    ptrs = (VP * len(fs))(*[planes[f].ctypes.data if f != fs[-1] else None for f in fs])
+    strides = (ctypes.c_ssize_t * len(fs))(*[planes[f].strides[0] for f in fs])
+    dst = np.zeros((80, 128), np.float32)
+    err = _lib.err_buf()
+    rc = L.vsfeel_bm3d_get_frame(node._handle, 2, ptrs, strides, None, None, dst.ctypes.data, dst.strides[0],
+                                 err, len(err))
+    assert rc == _lib.EINVAL and err.value.decode() == f"BM3D: missing plane of frame {fs[-1]}"
+    for n in (2, 0, 5, 3):
+        assert np.array_equal(node.get_frame(n)[0], want[n]), f"frame {n} differs after the failed request"

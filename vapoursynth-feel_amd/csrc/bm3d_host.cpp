@@ -1307,14 +1307,14 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
     // upload the window frames the ring does not hold (src/bm3d.cpp:772-814)
     // on the copy stream, each after the kernels that last read its slot;
     // requests that ran while this one staged may have filled a slot (its
-    // staged copy goes unused) or evicted a frame (staged now, under the mutex)
+    // staged copy goes unused) or evicted a frame (staged now, under the
+    // mutex).  Every frame is staged before the first upload is enqueued,
+    // so an input error leaves the ring as it was.
+    bool upload[4 * kMaxRadius + 1];
     for (int f = lo; f <= hi; ++f) {
-        const int slot = f % d.src_ring;
-        if (r > 0 && d.src_frame[slot] == f) {
-            continue;
-        }
         const int i = f - lo;
-        if (stage_of[i] < 0) {
+        upload[i] = r == 0 || d.src_frame[f % d.src_ring] != f;
+        if (upload[i] && stage_of[i] < 0) {
             if (!clip_planes[i] || (d.final_pass && !ref_planes[i])) {
                 return missing(f);
             }
@@ -1322,6 +1322,13 @@ int vsfeel_bm3d_get_frame(vsfeel_bm3d *h, int32_t n, const float *const *clip_pl
             VSFEEL_HIP(d.stage(o, stage_of[i], clip_planes[i], clip_strides[i],
                                d.final_pass ? ref_planes[i] : nullptr, d.final_pass ? ref_strides[i] : 0),
                        "hipHostMalloc");
+        }
+    }
+    for (int f = lo; f <= hi; ++f) {
+        const int slot = f % d.src_ring;
+        const int i = f - lo;
+        if (!upload[i]) {
+            continue;
         }
         if (!alone && d.slot_ev_set[size_t(slot)]) {
             VSFEEL_HIP(hipStreamWaitEvent(cin, d.slot_ev[size_t(slot)], 0), "hipStreamWaitEvent");
