@@ -155,21 +155,35 @@ struct Instance {
     float *res_buf = nullptr;    // res_cap stacks of tw * 2 planes
     uint32_t *groups = nullptr;  // nref * 8
     float *dists = nullptr;      // nref * 8 (debug)
-    // Radius 0, tiled, every window 8 candidates or more: the frames are
-    // independent and an estimate writes only its group table, tile
-    // accumulators and plane set beside its output, so process_device runs
-    // consecutive frames round-robin on kLanes streams, lane i > 0 with its
-    // own set of those buffers: one frame's filtering overlaps the next
-    // one's block matching (3 lanes: 2864 frames/s at 1080p against 2826
-    // with 2, 2653 with 1 and 2828 with 4, where the streams outnumber the
-    // 4 hardware queues).
-    static constexpr int kLanes = 3;
+    // Lanes (tiled, every window 8 candidates or more).  An estimate writes
+    // only its result slot (or, at radius 0, its output frame) and its
+    // intermediate buffers: group table, tile accumulators, plane set and at
+    // radius > 0 the spatial top-8, origin planes and binning records.  With
+    // a set of those per lane, process_device runs consecutive estimates
+    // round-robin on several streams, so one frame's filtering overlaps the
+    // next one's block matching.  Radius 0: kLanes lanes, lane 0 on the
+    // launch stream, lane i > 0 on stream_l[i - 1] (2864 frames/s at 1080p
+    // against 2826 with 2 lanes, 2653 with 1 and 2828 with 4, where the
+    // streams outnumber the 4 hardware queues).  Radius > 0: kTemporalLanes
+    // lanes, lane i on stream_l[i], the aggregations on the launch stream,
+    // each after the latest estimate of every lane; an estimate waits for
+    // the last aggregation that read the result slot it overwrites.
+    static constexpr int kLanes = 3, kTemporalLanes = 2;
+    static_assert(kTemporalLanes <= kLanes - 1, "radius > 0 puts every lane on a side stream");
     uint32_t *groups_l[kLanes - 1] = {};
     float *scratch_l[kLanes - 1] = {};
     uint16_t *psets_l[kLanes - 1] = {};
+    uint2 *spatial_l[kLanes - 1] = {};
+    float *origins_l[kLanes - 1] = {};
+    float *rec_l[kLanes - 1] = {};
+    uint32_t *cell_count_l[kLanes - 1] = {};
+    uint2 *cell_list_l[kLanes - 1] = {};
     hipStream_t stream_l[kLanes - 1] = {};
     hipEvent_t lane_fork = nullptr, lane_join[kLanes - 1] = {};
-    int lane = 0;  // the buffer set of the estimate being enqueued
+    hipEvent_t lane_done[kLanes] = {};       // radius > 0: a lane's latest estimate
+    std::vector<hipEvent_t> slot_read;       // radius > 0: last aggregation reading a result slot
+    int lanes = 1;   // lanes of process_device (1: everything on the launch stream)
+    int lane = 0;    // the buffer set of the estimate being enqueued
     uint2 *spatial = nullptr;    // nref * 8 spatial top-8 handed to the temporal search
 
     // row-parallel spatial search (bm3d_match_rows): ref blocks per
@@ -467,6 +481,11 @@ struct Instance {
             (void)hipFree(groups_l[i]);
             (void)hipFree(scratch_l[i]);
             (void)hipFree(psets_l[i]);
+            (void)hipFree(spatial_l[i]);
+            (void)hipFree(origins_l[i]);
+            (void)hipFree(rec_l[i]);
+            (void)hipFree(cell_count_l[i]);
+            (void)hipFree(cell_list_l[i]);
             if (lane_join[i]) {
                 (void)hipEventDestroy(lane_join[i]);
             }
@@ -476,6 +495,16 @@ struct Instance {
         }
         if (lane_fork) {
             (void)hipEventDestroy(lane_fork);
+        }
+        for (hipEvent_t e : lane_done) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        for (hipEvent_t e : slot_read) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
         }
         if (last_done) {
             (void)hipEventDestroy(last_done);
@@ -517,6 +546,11 @@ struct Instance {
     uint32_t *lane_groups() const { return lane ? groups_l[lane - 1] : groups; }
     float *lane_scratch() const { return lane ? scratch_l[lane - 1] : scratch; }
     uint16_t *lane_psets() const { return lane ? psets_l[lane - 1] : psets; }
+    uint2 *lane_spatial() const { return lane ? spatial_l[lane - 1] : spatial; }
+    float *lane_origins() const { return lane ? origins_l[lane - 1] : origins; }
+    float *lane_rec() const { return lane ? rec_l[lane - 1] : rec; }
+    uint32_t *lane_cell_count() const { return lane ? cell_count_l[lane - 1] : cell_count; }
+    uint2 *lane_cell_list() const { return lane ? cell_list_l[lane - 1] : cell_list; }
 
     // ------------------------------------------------------------------
     // one estimation dispatch (record_bm3d_kernels, src/bm3d.cpp:558-602):
@@ -540,7 +574,7 @@ struct Instance {
         m.ps_range = ps_range;
         m.nx = nx;
         m.ny = ny;
-        m.spatial = spatial;
+        m.spatial = lane_spatial();
         m.run_blocks = run_blocks;
         m.runs_x = runs_x;
         m.region_cols = region_cols;
@@ -592,6 +626,10 @@ struct Instance {
                         int mode = kFilterPlain, const Saved &saved = Saved(), float *fused_dst = nullptr,
                         int fused_pitch = 0)
     {
+        float *const origins = lane_origins();
+        float *const rec = lane_rec();
+        uint32_t *const cell_count = lane_cell_count();
+        uint2 *const cell_list = lane_cell_list();
         hipError_t e = hipSuccess;
         if (!tiled || radius > 0 || short_windows) {
             // the tile reduction overwrites the frame's own slice; the other
@@ -1162,21 +1200,6 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
             hipSuccess) {
             return dev_fail(e, "hipMemcpy");
         }
-        if (d.radius == 0 && !d.short_windows) {
-            if ((e = hipEventCreateWithFlags(&d.lane_fork, hipEventDisableTiming)) != hipSuccess) {
-                return dev_fail(e, "hipEventCreateWithFlags");
-            }
-            for (int i = 0; i < Instance::kLanes - 1; ++i) {
-                if ((e = hipMalloc(&d.groups_l[i], size_t(d.nref) * 8 * sizeof(uint32_t))) != hipSuccess ||
-                    (e = hipMalloc(&d.scratch_l[i], scratch_elems * sizeof(float))) != hipSuccess ||
-                    (d.operand != kOperandFp32 &&
-                     (e = hipMalloc(&d.psets_l[i], size_t(d.pset_bytes))) != hipSuccess) ||
-                    (e = hipStreamCreateWithFlags(&d.stream_l[i], hipStreamNonBlocking)) != hipSuccess ||
-                    (e = hipEventCreateWithFlags(&d.lane_join[i], hipEventDisableTiming)) != hipSuccess) {
-                    return dev_fail(e, "hipMalloc");
-                }
-            }
-        }
         if (d.radius > 0 && 2 * d.radius * d.ps_num >= 8) {
             // binning pays where a group takes at least 8 temporal
             // insertions (at 4K, radius 1, ps_num 2 the gather cost more
@@ -1211,6 +1234,49 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
                     (e = hipMalloc(&d.cell_count, size_t(d.cell_count_elems) * sizeof(uint32_t))) != hipSuccess ||
                     (e = hipMalloc(&d.cell_list, size_t(2LL * d.radius * d.ncells * d.cell_cap) * sizeof(uint2))) !=
                         hipSuccess) {
+                    return dev_fail(e, "hipMalloc");
+                }
+            }
+        }
+        if (!d.short_windows) {
+            // the lanes' own buffer sets (lane 0 has the instance's)
+            d.lanes = d.radius == 0 ? Instance::kLanes : Instance::kTemporalLanes;
+            const size_t rec_bytes = size_t((long long)d.tiles_x * d.tiles_y * d.rec_per_tile) * 64 * sizeof(float);
+            if ((e = hipEventCreateWithFlags(&d.lane_fork, hipEventDisableTiming)) != hipSuccess) {
+                return dev_fail(e, "hipEventCreate");
+            }
+            for (int i = 0; i < Instance::kLanes - 1; ++i) {
+                if ((e = hipStreamCreateWithFlags(&d.stream_l[i], hipStreamNonBlocking)) != hipSuccess ||
+                    (e = hipEventCreateWithFlags(&d.lane_join[i], hipEventDisableTiming)) != hipSuccess) {
+                    return dev_fail(e, "hipStreamCreate");
+                }
+            }
+            for (int i = 0; i < d.lanes; ++i) {
+                if ((e = hipEventCreateWithFlags(&d.lane_done[i], hipEventDisableTiming)) != hipSuccess) {
+                    return dev_fail(e, "hipEventCreate");
+                }
+            }
+            if (d.radius > 0) {
+                d.slot_read.assign(d.res_cap, nullptr);
+                for (hipEvent_t &ev : d.slot_read) {
+                    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+                        return dev_fail(e, "hipEventCreate");
+                    }
+                }
+            }
+            for (int i = 0; i < d.lanes - 1; ++i) {
+                if ((e = hipMalloc(&d.groups_l[i], size_t(d.nref) * 8 * sizeof(uint32_t))) != hipSuccess ||
+                    (e = hipMalloc(&d.scratch_l[i], scratch_elems * sizeof(float))) != hipSuccess ||
+                    (d.operand != kOperandFp32 &&
+                     (e = hipMalloc(&d.psets_l[i], size_t(d.pset_bytes))) != hipSuccess) ||
+                    (d.spatial && (e = hipMalloc(&d.spatial_l[i], size_t(d.nref) * 8 * sizeof(uint2))) != hipSuccess) ||
+                    (d.origins && (e = hipMalloc(&d.origins_l[i], plane * 2 * d.radius)) != hipSuccess) ||
+                    (d.rec && (e = hipMalloc(&d.rec_l[i], rec_bytes)) != hipSuccess) ||
+                    (d.rec && (e = hipMalloc(&d.cell_count_l[i], size_t(d.cell_count_elems) * sizeof(uint32_t))) !=
+                                  hipSuccess) ||
+                    (d.rec && (e = hipMalloc(&d.cell_list_l[i],
+                                             size_t(2LL * d.radius * d.ncells * d.cell_cap) * sizeof(uint2))) !=
+                                  hipSuccess)) {
                     return dev_fail(e, "hipMalloc");
                 }
             }
@@ -1583,12 +1649,13 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
     const int r = d.radius, nf = d.nframes;
     // the whole call's launch sequence; estimate stacks are only reused
     // inside one call: the caller owns the clip memory and may rewrite it
-    // between calls
-    // with the second buffer set (radius 0) the frames alternate between the
-    // launch stream and the instance's second stream, forked from it and
-    // joined back at the end (not under per-kernel timing, which times each
-    // kernel alone)
-    const int nlanes = d.lane_fork != nullptr && !d.timing ? std::min(count, (int)Instance::kLanes) : 1;
+    // between calls.  The k-th estimate of the call runs on lane k mod
+    // nlanes (see Instance::lanes; everything on the launch stream under
+    // per-kernel timing, which times each kernel alone).  Side streams are
+    // forked from the launch stream when first used and joined back at the
+    // end.
+    const int nlanes = d.timing ? 1 : d.lanes;
+    const bool fused = d.fuses();
     auto enqueue = [&](hipStream_t q) -> hipError_t {
         struct LaneReset {
             Instance &d;
@@ -1596,54 +1663,80 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
         } lane_reset{d};
         std::vector<int> held(d.res_cap, -1);
         std::vector<int> slot_of(d.tw);
+        std::vector<char> slot_read(d.res_cap, 0);  // slot_read[slot] recorded in this call
+        bool forked[Instance::kLanes] = {}, fresh[Instance::kLanes] = {};
+        // lane j's index in stream_l (-1: the launch stream)
+        auto side = [&](int j) { return nlanes == 1 ? -1 : fused ? j - 1 : j; };
+        auto lane_stream = [&](int j) { return side(j) < 0 ? q : d.stream_l[side(j)]; };
+        hipError_t e = hipSuccess;
         if (nlanes > 1) {
-            hipError_t e = hipEventRecord(d.lane_fork, q);
-            for (int i = 0; i < nlanes - 1 && e == hipSuccess; ++i) {
-                e = hipStreamWaitEvent(d.stream_l[i], d.lane_fork, 0);
-            }
-            if (e != hipSuccess) {
-                return e;
-            }
+            e = hipEventRecord(d.lane_fork, q);
         }
-        for (int n = first; n < first + count; ++n) {
-            d.lane = (n - first) % nlanes;
-            const hipStream_t qn = d.lane ? d.stream_l[d.lane - 1] : q;
-            for (int i = 0; i < d.tw; ++i) {
+        int k = 0;
+        for (int n = first; n < first + count && e == hipSuccess; ++n) {
+            for (int i = 0; i < d.tw && e == hipSuccess; ++i) {
                 const int m = clampi(n - r + i, 0, nf - 1);
                 const int slot = m % d.res_cap;
                 slot_of[i] = slot;
                 if (held[slot] == m) {
                     continue;
                 }
+                d.lane = k++ % nlanes;
+                const hipStream_t qs = lane_stream(d.lane);
+                if (qs != q && !forked[d.lane]) {
+                    e = hipStreamWaitEvent(qs, d.lane_fork, 0);
+                    forked[d.lane] = true;
+                }
+                if (e == hipSuccess && nlanes > 1 && !fused && slot_read[slot]) {
+                    e = hipStreamWaitEvent(qs, d.slot_read[slot], 0);
+                }
+                if (e != hipSuccess) {
+                    break;
+                }
                 const float *search[kMaxWindow];
                 const float *input[kMaxWindow];
                 device_window(d, clip, m, search, input);
                 // radius 0: the tile reduction writes the frame (fused aggregation)
-                hipError_t e = d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, qn, nullptr,
-                                          kFilterPlain, Instance::Saved(),
-                                          out + (int64_t)(n - first) * out_frame_stride, out_pitch);
-                if (e != hipSuccess) {
-                    return e;
+                e = d.estimate(search, input, clip->pitch, d.res_slot(slot), d.pitch, qs, nullptr, kFilterPlain,
+                               Instance::Saved(), out + (int64_t)(n - first) * out_frame_stride, out_pitch);
+                if (e == hipSuccess && nlanes > 1 && !fused) {
+                    e = hipEventRecord(d.lane_done[d.lane], qs);
+                    fresh[d.lane] = true;
                 }
                 held[slot] = m;
             }
-            if (!d.fuses()) {
-                hipError_t e = d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, q);
-                if (e != hipSuccess) {
-                    return e;
+            if (fused || e != hipSuccess) {
+                continue;
+            }
+            for (int j = 0; j < nlanes && e == hipSuccess; ++j) {
+                if (fresh[j]) {
+                    e = hipStreamWaitEvent(q, d.lane_done[j], 0);
+                    fresh[j] = false;
+                }
+            }
+            if (e == hipSuccess) {
+                e = d.aggregate(n, slot_of, out + (int64_t)(n - first) * out_frame_stride, out_pitch, q);
+            }
+            for (int i = 0; i < d.tw && e == hipSuccess && nlanes > 1; ++i) {
+                e = hipEventRecord(d.slot_read[slot_of[i]], q);
+                slot_read[slot_of[i]] = 1;
+            }
+        }
+        // join every side stream, also after an error (a capture must end
+        // with its forks joined)
+        for (int j = 0; j < nlanes; ++j) {
+            if (forked[j]) {
+                const int li = side(j);
+                hipError_t ej = hipEventRecord(d.lane_join[li], d.stream_l[li]);
+                if (ej == hipSuccess) {
+                    ej = hipStreamWaitEvent(q, d.lane_join[li], 0);
+                }
+                if (e == hipSuccess) {
+                    e = ej;
                 }
             }
         }
-        for (int i = 0; i < nlanes - 1; ++i) {
-            hipError_t e = hipEventRecord(d.lane_join[i], d.stream_l[i]);
-            if (e == hipSuccess) {
-                e = hipStreamWaitEvent(q, d.lane_join[i], 0);
-            }
-            if (e != hipSuccess) {
-                return e;
-            }
-        }
-        return hipSuccess;
+        return e;
     };
     if (d.graphs && !d.timing && count > 0) {
         // replay from a graph captured on the instance stream, ordered after
