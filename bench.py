@@ -410,11 +410,20 @@ def main():
         frames = 300
         pipe_fps = pipe_bench.run(frames=frames, synthetic=False, threads=4)
         pipe_graphs = pipe_bench.run(frames=frames, synthetic=False, threads=4, graphs=True)
+        # radius 0 from 8 threads with 8 output slots: the host path against
+        # the device-resident headline rate
+        pipe_r0 = pipe_bench.run(frames=2 * frames, synthetic=False, threads=8, radius=0, sigma=3.0,
+                                 num_streams=8)
         extra["pipe"] = dict(value=pipe_fps, unit="frames/s", graph_launches=pipe_graphs,
                              workload=f"{frames} frames of 1920x1080 GrayS float32 (16 synthetic frames cycled), "
                                       "host buffers from 4 threads (PCIe-inclusive), sigma=0.7, block_step=4, "
-                                      "bm_range=16, radius=2, ps_range=7, num_streams=4; value from "
-                                      "per-request graphs, direct_launches without")
+                                      "bm_range=16, radius=2, ps_range=7, num_streams=4; value with direct "
+                                      "launches, graph_launches with per-request graphs",
+                             radius0_8threads=dict(
+                                 value=pipe_r0, unit="frames/s",
+                                 workload=f"{2 * frames} frames of 1920x1080 GrayS float32, host buffers from 8 "
+                                          "threads (PCIe-inclusive), sigma=3.0, block_step=4, bm_range=16, "
+                                          "radius=0, num_streams=8, direct launches"))
     if "c4" in forms:
         extra["c4"] = bench_c4(max(1, args.steps // 2), 1, max(1, min(16, os.cpu_count() or 1)))
     tsteps = max(1, args.steps // 2)

@@ -793,7 +793,9 @@ __device__ TopList predictive_fallback(int h, const uint3 &m, bool row_ok, uint3
     return L;
 }
 
-template <int CM>
+// PNC, PRC > 0: ps_num and ps_range fixed at compile time (the window
+// geometry then folds into the LDS offsets); 0: taken from the arguments
+template <int CM, int PNC, int PRC>
 __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
 {
     extern __shared__ __align__(16) unsigned long long plds[];
@@ -807,7 +809,7 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
     const int W = A.width, H = A.height, pitch = A.pitch, r = A.radius;
     const int x = min(bx * A.step, W - 8);
     const int y = min(by * A.step, H - 8);
-    const int PR = A.ps_range, PN = A.ps_num;
+    const int PR = PRC > 0 ? PRC : A.ps_range, PN = PNC > 0 ? PNC : A.ps_num;
     const int wh = 2 * PR + 1, win = wh * wh, wc = wh + 7;
     const uint32_t pm = pos_mask(wh);
     unsigned long long *sk = plds;                                // [2][32] survivor keys
@@ -923,7 +925,13 @@ __global__ __launch_bounds__(64) void bm3d_match_predictive(MatchArgs A)
         float *drow = tdist + lane * wh;
         uint3 m = make_uint3(~0u, ~0u, ~0u);
         if (row_ok) {
-            m = scan_row_sliding<CM, 5>(cur, 8, regions + (half * PN + w) * WS + rr, wc, rw, drow, pm);
+            // the row's offset from the start of LDS, opaque to the compiler:
+            // it would otherwise split the regions' constant base back out
+            // into the immediate offsets, which then overflow ds_read2_b32's
+            // 8-bit fields and cost an address add per pair of reads
+            int row_off = int(regions - reinterpret_cast<float *>(plds)) + (half * PN + w) * WS + rr;
+            asm volatile("" : "+v"(row_off));
+            m = scan_row_sliding<CM, 5>(cur, 8, reinterpret_cast<float *>(plds) + row_off, wc, rw, drow, pm);
         }
         wave_lds_sync();
         // only the step's first ps_num (distance, seq) keys are used (the
@@ -2025,12 +2033,21 @@ static hipError_t launch_temporal(const MatchArgs &a, int fp_mode, hipStream_t s
     const bool fused = fp_mode == kFpAll;
     if (a.pred_pair) {
         const size_t plds = size_t(predictive_lds_bytes(a.ps_num, a.ps_range));
+        // compiled window geometries: the plugin's default ps_range 4 and
+        // the benchmark harness's 7, both with the default ps_num 2
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(nref), dim3(64), plds, s, a);
+            return hipGetLastError();
+        };
+        const int geo = a.ps_num != 2 ? 0 : a.ps_range == 4 ? 4 : a.ps_range == 7 ? 7 : 0;
         if (fused) {
-            hipLaunchKernelGGL((bm3d_match_predictive<kFpAll>), dim3(nref), dim3(64), plds, s, a);
-        } else {
-            hipLaunchKernelGGL((bm3d_match_predictive<kFpExplicit>), dim3(nref), dim3(64), plds, s, a);
+            return geo == 4 ? go(bm3d_match_predictive<kFpAll, 2, 4>)
+                   : geo == 7 ? go(bm3d_match_predictive<kFpAll, 2, 7>)
+                              : go(bm3d_match_predictive<kFpAll, 0, 0>);
         }
-        return hipGetLastError();
+        return geo == 4 ? go(bm3d_match_predictive<kFpExplicit, 2, 4>)
+               : geo == 7 ? go(bm3d_match_predictive<kFpExplicit, 2, 7>)
+                          : go(bm3d_match_predictive<kFpExplicit, 0, 0>);
     }
     const size_t tlds = size_t(temporal_lds_bytes(a.t_rows, a.ps_range));
     if (fused) {
