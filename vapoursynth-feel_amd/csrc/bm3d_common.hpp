@@ -140,6 +140,20 @@ __host__ __device__ inline long long predictive_lds_bytes(int ps_num, int ps_ran
 // that the staging stores (one column per lane) fall in distinct LDS banks
 __host__ __device__ inline int region_col_stride(int region_rows) { return region_rows | 1; }
 
+// Run geometry of bm3d_match_rows for a frame at least as large as the run's
+// region (the window's 2 bm_range + 1 rows and columns, and the run's
+// columns, inside the frame): 256 window rows per workgroup, so
+// 256 / (2 bm_range + 1) blocks per run.  Zero for bm_range <= 0.
+struct RowsGeometry {
+    int run_blocks, region_cols, region_rows, dist_cols;
+};
+__host__ __device__ constexpr RowsGeometry rows_geometry(int bm_range, int step)
+{
+    return bm_range > 0 ? RowsGeometry{256 / (2 * bm_range + 1), (256 / (2 * bm_range + 1) - 1) * step + 8 + 2 * bm_range,
+                                       2 * bm_range + 8, 2 * bm_range + 1}
+                        : RowsGeometry{0, 0, 0, 0};
+}
+
 __host__ __device__ inline long long rows_lds_bytes(int region_cols, int region_rows, int run_blocks, int dist_cols)
 {
     // staged positions, per-block flags and thresholds, 3 row keys per

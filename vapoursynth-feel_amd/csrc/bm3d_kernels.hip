@@ -567,23 +567,31 @@ __device__ __forceinline__ TopList select_predictive(const float *tdist, uint32_
 // recomputes their distances exactly and sorts them (select16); blocks with
 // more than 16 such candidates go to a whole-wavefront pass (select_wave).
 // ---------------------------------------------------------------------------
-template <int CM>
+// BRC, STC > 0: bm_range and block_step fixed at compile time, with the run
+// geometry that the host derives from them for frames at least as large as
+// the run's region (rows_geometry); 0: taken from the arguments
+template <int CM, int BRC, int STC>
 __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
 {
     extern __shared__ __align__(16) unsigned long long lds64[];
-    const int NB = A.run_blocks;
-    const int CS = region_col_stride(A.region_rows);  // column stride of the region
-    const int RH = A.region_rows - 7;
+    constexpr RowsGeometry G = rows_geometry(BRC, STC);
+    const int NB = BRC > 0 ? G.run_blocks : A.run_blocks;
+    const int RROWS = BRC > 0 ? G.region_rows : A.region_rows;
+    const int RCOLS = BRC > 0 ? G.region_cols : A.region_cols;
+    const int DCOLS = BRC > 0 ? G.dist_cols : A.dist_cols;
+    const int CS = region_col_stride(RROWS);  // column stride of the region
+    const int RH = RROWS - 7;
     uint32_t *stage_xy = reinterpret_cast<uint32_t *>(lds64);               // [4][64] staged positions
     int *stage_over = reinterpret_cast<int *>(stage_xy + 256);              // [NB] more than 16 staged
     uint32_t *stage_thr = reinterpret_cast<uint32_t *>(stage_over + NB);    // [NB] their threshold
     // per window row: its 3 smallest row keys (x <= y <= z)
     uint32_t *rowtop = stage_thr + NB;                                      // [NB][rh][3]
     float *region = reinterpret_cast<float *>(rowtop + 3 * NB * RH);        // [region_cols][CS]
-    uint16_t *dist = reinterpret_cast<uint16_t *>(region + A.region_cols * CS);  // [NB][rh][dist_cols]
-    const uint32_t pm = pos_mask(A.dist_cols);
+    uint16_t *dist = reinterpret_cast<uint16_t *>(region + RCOLS * CS);  // [NB][rh][dist_cols]
+    const uint32_t pm = pos_mask(DCOLS);
 
-    const int W = A.width, H = A.height, pitch = A.pitch, R = A.bm_range, s = A.step;
+    const int W = A.width, H = A.height, pitch = A.pitch;
+    const int R = BRC > 0 ? BRC : A.bm_range, s = STC > 0 ? STC : A.step;
     const int run = xcd_block_id();
     const int by = run / A.runs_x;
     const int bx0 = (run - by * A.runs_x) * NB;
@@ -637,7 +645,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         // the reference block lies in the region: column c at cb + c * CS
         const float *cb = region + (xb - rx0) * CS + (y - top);
         const uint3 t3 = scan_row_sliding<CM, 11>(cb, CS, region + (left - rx0) * CS + r, CS, right - left + 1,
-                                              dist + (b * rh + r) * A.dist_cols, pm);
+                                              dist + (b * rh + r) * DCOLS, pm);
         rowtop[3 * (b * rh + r)] = t3.x;
         rowtop[3 * (b * rh + r) + 1] = t3.y;
         rowtop[3 * (b * rh + r) + 2] = t3.z;
@@ -666,7 +674,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         uint64_t key;
         uint32_t kxy;
         uint32_t thr;
-        const int staged = select16(rowtop + 3 * bs * rh, dist + bs * rh * A.dist_cols, A.dist_cols, rh, geo, act,
+        const int staged = select16(rowtop + 3 * bs * rh, dist + bs * rh * DCOLS, DCOLS, rh, geo, act,
                                     lane, stage_xy + wave * 64 + grp * 16, key, kxy, thr, pm);
         if (act && staged <= 16) {
             if (A.spatial) {
@@ -696,7 +704,7 @@ __global__ __launch_bounds__(256) void bm3d_match_rows(MatchArgs A)
         const SpatialRows<CM> geo{right - left + 1, left, top, region + (xb - rx0) * CS + (y - top),
                                   region + (left - rx0) * CS, CS};
         TopList L;
-        select_wave(rowtop + 3 * b * rh, dist + b * rh * A.dist_cols, A.dist_cols, rh, geo, stage_thr[b], pm, lane,
+        select_wave(rowtop + 3 * b * rh, dist + b * rh * DCOLS, DCOLS, rh, geo, stage_thr[b], pm, lane,
                     stage_xy + wave * 64, L);
         if (A.spatial) {
             if (lane < 8) {
@@ -2087,12 +2095,27 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s, bool tem
         }
         const size_t lds = size_t(rows_lds_bytes(a.region_cols, a.region_rows, a.run_blocks, a.dist_cols));
         const dim3 rgrid(a.ny * a.runs_x);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, rgrid, block, lds, s, rows);
+            return hipGetLastError();
+        };
+        // compiled run geometries: bm_range 16 at block_step 4 (the
+        // benchmark harness) and bm_range 9 at block_step 8 (the plugin's
+        // defaults), where the host derived exactly that geometry
+        auto is = [&](int br, int st) {
+            const RowsGeometry g = rows_geometry(br, st);
+            return a.bm_range == br && a.step == st && a.run_blocks == g.run_blocks &&
+                   a.region_cols == g.region_cols && a.region_rows == g.region_rows && a.dist_cols == g.dist_cols;
+        };
+        const int geo = is(16, 4) ? 1 : is(9, 8) ? 2 : 0;
         if (fused) {
-            hipLaunchKernelGGL((bm3d_match_rows<kFpAll>), rgrid, block, lds, s, rows);
-        } else {
-            hipLaunchKernelGGL((bm3d_match_rows<kFpExplicit>), rgrid, block, lds, s, rows);
+            return geo == 1 ? go(bm3d_match_rows<kFpAll, 16, 4>)
+                   : geo == 2 ? go(bm3d_match_rows<kFpAll, 9, 8>)
+                              : go(bm3d_match_rows<kFpAll, 0, 0>);
         }
-        return hipGetLastError();
+        return geo == 1 ? go(bm3d_match_rows<kFpExplicit, 16, 4>)
+               : geo == 2 ? go(bm3d_match_rows<kFpExplicit, 9, 8>)
+                          : go(bm3d_match_rows<kFpExplicit, 0, 0>);
     }
     MatchArgs b = a;
     b.spatial = nullptr;
