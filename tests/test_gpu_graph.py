@@ -122,3 +122,32 @@ def test_host_requests_from_graphs_match_direct_launches(radius, extractor_exp):
     # switching request graphs off on a live node drops the slots' graphs
     assert L.vsfeel_bm3d_set_graphs(node._handle, 1) == _lib.OK
     assert np.array_equal(node.get_frame(3)[0], want[3])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_lane_count_leaves_frames_unchanged(name, monkeypatch):
+    # process_device runs consecutive estimates on up to three lanes (own
+    # streams and buffer sets); VSFEEL_BM3D_LANES=1, read when the instance
+    # is created, keeps everything on one stream: the frames are identical
+    torch = _torch()
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+
+    make, kw = CASES[name]
+    clip = np.ascontiguousarray(make(synth), np.float32)
+    nf, h, w = clip.shape
+    print(f"lanes case {name} clip={clip.shape}", flush=True)
+    x = torch.from_numpy(clip).to("cuda:0")
+    got = {}
+    for lanes in ("1", "2", None):
+        if lanes is None:
+            monkeypatch.delenv("VSFEEL_BM3D_LANES", raising=False)
+        else:
+            monkeypatch.setenv("VSFEEL_BM3D_LANES", lanes)
+        inst = Instance(nf, h, w, **kw)
+        for graphs in (False, True):
+            inst.set_graphs(graphs)
+            got[(lanes, graphs)] = inst.process(x).cpu().numpy()
+    want = got[("1", False)]
+    for key, frames in got.items():
+        assert np.array_equal(frames, want), f"{name}: lanes={key[0]} graphs={key[1]} differs"

@@ -1238,9 +1238,15 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
                 }
             }
         }
-        if (!d.short_windows) {
+        // VSFEEL_BM3D_LANES caps the lanes (1: one stream and no extra
+        // buffer sets, e.g. for per-kernel profiles of kernels running alone)
+        int max_lanes = Instance::kLanes;
+        if (const char *l = std::getenv("VSFEEL_BM3D_LANES")) {
+            max_lanes = std::min(std::max(std::atoi(l), 1), Instance::kLanes);
+        }
+        if (!d.short_windows && max_lanes > 1) {
             // the lanes' own buffer sets (lane 0 has the instance's)
-            d.lanes = d.radius == 0 ? Instance::kLanes : Instance::kTemporalLanes;
+            d.lanes = std::min(d.radius == 0 ? Instance::kLanes : Instance::kTemporalLanes, max_lanes);
             const size_t rec_bytes = size_t((long long)d.tiles_x * d.tiles_y * d.rec_per_tile) * 64 * sizeof(float);
             if ((e = hipEventCreateWithFlags(&d.lane_fork, hipEventDisableTiming)) != hipSuccess) {
                 return dev_fail(e, "hipEventCreate");
