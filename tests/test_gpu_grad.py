@@ -187,3 +187,30 @@ def test_full_size_batch_runs_and_is_finite():
     (out2 * (2.0 * g)).sum().backward()
     torch.cuda.synchronize()
     assert torch.allclose(x2.grad, 2.0 * x.grad, rtol=1e-5, atol=1e-6)
+
+
+def test_frame_lanes_leave_saved_state_and_gradient_unchanged(monkeypatch):
+    # forward_saved and backward run a batch's frames on up to three lanes
+    # (own streams, result slots and buffers); with VSFEEL_BM3D_LANES=1 they
+    # run one after another: every output, saved tensor and gradient is
+    # bit-identical
+    torch = _torch()
+    from vsfeel import synth
+    from vsfeel.torch_bm3d import Instance
+    clip = synth.workload(5, frames=7, width=160, height=128)
+    g = torch.from_numpy(np.random.default_rng(4).standard_normal(clip.shape).astype(np.float32)).to("cuda:0")
+    x = torch.from_numpy(clip).to("cuda:0")
+    print(f"frame lanes clip={clip.shape}", flush=True)
+    got = {}
+    for lanes in ("1", None):
+        if lanes is None:
+            monkeypatch.delenv("VSFEEL_BM3D_LANES", raising=False)
+        else:
+            monkeypatch.setenv("VSFEEL_BM3D_LANES", lanes)
+        inst = Instance(*clip.shape, radius=0, **KW)
+        out, saved = inst.forward_saved(x)
+        gin = inst.backward(g, saved)
+        torch.cuda.synchronize()
+        got[lanes] = [t.cpu().numpy() for t in (out, *saved, gin)]
+    for name, a, b in zip(("out", "groups", "masks", "weights", "denom", "grad"), got["1"], got[None]):
+        assert np.array_equal(a, b), f"{name} differs between one and three lanes"

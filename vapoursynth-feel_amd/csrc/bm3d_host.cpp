@@ -178,6 +178,7 @@ struct Instance {
     float *rec_l[kLanes - 1] = {};
     uint32_t *cell_count_l[kLanes - 1] = {};
     uint2 *cell_list_l[kLanes - 1] = {};
+    float *dst_l[kLanes - 1] = {};  // radius 0: the backward pass's g / D plane
     hipStream_t stream_l[kLanes - 1] = {};
     hipEvent_t lane_fork = nullptr, lane_join[kLanes - 1] = {};
     hipEvent_t lane_done[kLanes] = {};       // radius > 0: a lane's latest estimate
@@ -486,6 +487,7 @@ struct Instance {
             (void)hipFree(rec_l[i]);
             (void)hipFree(cell_count_l[i]);
             (void)hipFree(cell_list_l[i]);
+            (void)hipFree(dst_l[i]);
             if (lane_join[i]) {
                 (void)hipEventDestroy(lane_join[i]);
             }
@@ -551,6 +553,7 @@ struct Instance {
     float *lane_rec() const { return lane ? rec_l[lane - 1] : rec; }
     uint32_t *lane_cell_count() const { return lane ? cell_count_l[lane - 1] : cell_count; }
     uint2 *lane_cell_list() const { return lane ? cell_list_l[lane - 1] : cell_list; }
+    float *lane_dst() const { return lane ? dst_l[lane - 1] : dst_dev; }
 
     // ------------------------------------------------------------------
     // one estimation dispatch (record_bm3d_kernels, src/bm3d.cpp:558-602):
@@ -781,6 +784,52 @@ struct Instance {
         a.tw = tw;
         return timed_launch(kKernelAggregate, s, [&] { return launch_aggregate(a, s); });
     }
+
+    // Frames of a radius-0 batch (forward_saved, backward) are independent:
+    // frame i runs on lane i mod n with the lane's own result slot (slot i
+    // mod n), buffers and stream; side streams are forked from the launch
+    // stream when first used and joined back by finish().  Under per-kernel
+    // timing everything stays on the launch stream.
+    struct FrameLanes {
+        Instance &d;
+        hipStream_t q;
+        int n = 1;
+        bool forked[kLanes] = {};
+        FrameLanes(Instance &inst, hipStream_t s) : d(inst), q(s)
+        {
+            n = d.timing || d.radius != 0 ? 1 : std::min(d.lanes, d.res_cap);
+        }
+        ~FrameLanes() { d.lane = 0; }
+        hipError_t start() { return n > 1 ? hipEventRecord(d.lane_fork, q) : hipSuccess; }
+        // selects frame i's lane; its stream in *qs
+        hipError_t select(int i, hipStream_t *qs)
+        {
+            d.lane = i % n;
+            *qs = d.lane ? d.stream_l[d.lane - 1] : q;
+            if (d.lane && !forked[d.lane]) {
+                forked[d.lane] = true;
+                return hipStreamWaitEvent(*qs, d.lane_fork, 0);
+            }
+            return hipSuccess;
+        }
+        hipError_t finish()
+        {
+            d.lane = 0;
+            hipError_t e = hipSuccess;
+            for (int j = 1; j < n; ++j) {
+                if (forked[j]) {
+                    hipError_t ej = hipEventRecord(d.lane_join[j - 1], d.stream_l[j - 1]);
+                    if (ej == hipSuccess) {
+                        ej = hipStreamWaitEvent(q, d.lane_join[j - 1], 0);
+                    }
+                    if (e == hipSuccess) {
+                        e = ej;
+                    }
+                }
+            }
+            return e;
+        }
+    };
 
     // order this call's device work after the previous call's
     hipError_t begin_on(hipStream_t s)
@@ -1282,7 +1331,8 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
                                   hipSuccess) ||
                     (d.rec && (e = hipMalloc(&d.cell_list_l[i],
                                              size_t(2LL * d.radius * d.ncells * d.cell_cap) * sizeof(uint2))) !=
-                                  hipSuccess)) {
+                                  hipSuccess) ||
+                    (d.radius == 0 && (e = hipMalloc(&d.dst_l[i], plane)) != hipSuccess)) {
                     return dev_fail(e, "hipMalloc");
                 }
             }
@@ -1910,9 +1960,15 @@ int vsfeel_bm3d_forward_saved(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cli
     VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
-    const std::vector<int> slot_of(1, 0);
+    // (the result slots are overwritten: the host path's cache forgets them)
+    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    Instance::FrameLanes fl(d, s);
+    VSFEEL_HIP(fl.start(), "hipEventRecord");
     for (int i = 0; i < count; ++i) {
         const int n = first + i;
+        hipStream_t qs = s;
+        hipError_t e = fl.select(i, &qs);
+        const std::vector<int> slot_of(1, d.lane);
         const float *search[kMaxWindow];
         const float *input[kMaxWindow];
         device_window(d, clip, n, search, input);
@@ -1920,13 +1976,19 @@ int vsfeel_bm3d_forward_saved(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cli
         sv.groups = saved->groups + (int64_t)i * d.nref * 8;
         sv.masks = saved->masks + (int64_t)i * d.nref * 8;
         sv.weights = saved->weights + (int64_t)i * d.nref;
-        VSFEEL_HIP(d.estimate(search, input, clip->pitch, d.res_slot(0), d.pitch, s, nullptr, kFilterSave, sv),
-                   "estimate launch");
-        VSFEEL_HIP(d.aggregate(n, slot_of, out + (int64_t)i * out_frame_stride, out_pitch, s,
-                               saved->denom + (int64_t)i * d.height * saved->denom_pitch, saved->denom_pitch),
-                   "aggregate launch");
+        if (e == hipSuccess) {
+            e = d.estimate(search, input, clip->pitch, d.res_slot(d.lane), d.pitch, qs, nullptr, kFilterSave, sv);
+        }
+        if (e == hipSuccess) {
+            e = d.aggregate(n, slot_of, out + (int64_t)i * out_frame_stride, out_pitch, qs,
+                            saved->denom + (int64_t)i * d.height * saved->denom_pitch, saved->denom_pitch);
+        }
+        if (e != hipSuccess) {
+            (void)fl.finish();
+            VSFEEL_HIP(e, "forward launch");
+        }
     }
-    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    VSFEEL_HIP(fl.finish(), "hipStreamWaitEvent");
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
     return VSFEEL_BM3D_OK;
 }
@@ -1951,24 +2013,38 @@ int vsfeel_bm3d_backward(vsfeel_bm3d *h, const float *grad_out, int32_t grad_pit
     VSFEEL_HIP(hipSetDevice(d.device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     VSFEEL_HIP(d.begin_on(s), "hipStreamWaitEvent");
-    const std::vector<int> slot_of(1, 0);
+    // (the result slots are overwritten: the host path's cache forgets them)
+    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    Instance::FrameLanes fl(d, s);
+    VSFEEL_HIP(fl.start(), "hipEventRecord");
     for (int i = 0; i < count; ++i) {
         // y = g / D, then grad = A y with the saved groups, masks and weights
-        VSFEEL_HIP(launch_divide(grad_out + (int64_t)i * grad_frame_stride, grad_pitch,
-                                 saved->denom + (int64_t)i * d.height * saved->denom_pitch, saved->denom_pitch,
-                                 d.dst_dev, d.pitch, d.width, d.height, s),
-                   "divide launch");
-        const float *y[kMaxWindow] = { d.dst_dev };
+        hipStream_t qs = s;
+        hipError_t e = fl.select(i, &qs);
+        const std::vector<int> slot_of(1, d.lane);
+        float *const yp = d.lane_dst();
+        if (e == hipSuccess) {
+            e = launch_divide(grad_out + (int64_t)i * grad_frame_stride, grad_pitch,
+                              saved->denom + (int64_t)i * d.height * saved->denom_pitch, saved->denom_pitch, yp,
+                              d.pitch, d.width, d.height, qs);
+        }
+        const float *y[kMaxWindow] = { yp };
         Instance::Saved sv;
         sv.groups = saved->groups + (int64_t)i * d.nref * 8;
         sv.masks = saved->masks + (int64_t)i * d.nref * 8;
         sv.weights = saved->weights + (int64_t)i * d.nref;
-        VSFEEL_HIP(d.estimate(y, y, d.pitch, d.res_slot(0), d.pitch, s, nullptr, kFilterApply, sv),
-                   "estimate launch");
-        VSFEEL_HIP(d.aggregate(0, slot_of, grad_in + (int64_t)i * gin_frame_stride, gin_pitch, s, nullptr, 0, true),
-                   "aggregate launch");
+        if (e == hipSuccess) {
+            e = d.estimate(y, y, d.pitch, d.res_slot(d.lane), d.pitch, qs, nullptr, kFilterApply, sv);
+        }
+        if (e == hipSuccess) {
+            e = d.aggregate(0, slot_of, grad_in + (int64_t)i * gin_frame_stride, gin_pitch, qs, nullptr, 0, true);
+        }
+        if (e != hipSuccess) {
+            (void)fl.finish();
+            VSFEEL_HIP(e, "backward launch");
+        }
     }
-    std::fill(d.res_frame.begin(), d.res_frame.end(), -1);
+    VSFEEL_HIP(fl.finish(), "hipStreamWaitEvent");
     VSFEEL_HIP(d.end_on(s), "hipEventRecord");
     return VSFEEL_BM3D_OK;
 }
