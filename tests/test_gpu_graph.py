@@ -124,7 +124,17 @@ def test_host_requests_from_graphs_match_direct_launches(radius, extractor_exp):
     assert np.array_equal(node.get_frame(3)[0], want[3])
 
 
-@pytest.mark.parametrize("name", list(CASES))
+# more frames than lanes, so that every lane's buffers are reused
+LANE_CASES = {
+    "spatial": (lambda s: s.workload(1, frames=7),
+                dict(sigma=3.0, block_step=8, bm_range=9, radius=0)),
+    "temporal": (lambda s: s.workload(3, frames=7, width=192, height=112)[0],
+                 dict(sigma=3.0, block_step=4, bm_range=16, radius=2, ps_num=2, ps_range=7,
+                      extractor_exp=2)),
+}
+
+
+@pytest.mark.parametrize("name", list(LANE_CASES))
 def test_lane_count_leaves_frames_unchanged(name, monkeypatch):
     # process_device runs consecutive estimates on up to three lanes (own
     # streams and buffer sets); VSFEEL_BM3D_LANES=1, read when the instance
@@ -133,7 +143,7 @@ def test_lane_count_leaves_frames_unchanged(name, monkeypatch):
     from vsfeel import synth
     from vsfeel.torch_bm3d import Instance
 
-    make, kw = CASES[name]
+    make, kw = LANE_CASES[name]
     clip = np.ascontiguousarray(make(synth), np.float32)
     nf, h, w = clip.shape
     print(f"lanes case {name} clip={clip.shape}", flush=True)
