@@ -93,35 +93,57 @@ enum FailStage { kFailNone = 0, kFailUpload = 1, kFailLaunch = 2 };
 constexpr int kTileLdsBudget = 112 * 1024;
 constexpr int kTileGroupsPerPass = 64;
 
+// Compute units of the MI355X: the tile choice below is fixed by it, not by
+// the device the instance runs on, so the tiling (and the last bit of the
+// float partial sums) never depends on the GPU.
+constexpr int kComputeUnits = 256;
+
 // Tile shape of the tiled estimate: the most ref blocks per workgroup whose
-// accumulator fits the LDS budget, preferring whole passes of 32 groups.
+// accumulator fits the LDS budget, preferring whole passes of 64 groups.
+// A frame that this leaves with fewer tiles than compute units (a 512x512
+// frame at block_step 4: 88 tiles) gets the shape that minimises the
+// launch's rounds of tiles times a tile's cost instead (256 tiles there),
+// so that the grid fills the GPU.
 bool choose_tiles(int width, int height, int step, int range, int *tbx, int *tby, int *rw, int *rh)
 {
-    double best = 1e30;
-    bool found = false;
-    for (int by = 1; by <= 32; ++by) {
-        for (int bx = 1; bx <= 32; ++bx) {
-            const int w = std::min((bx - 1) * step + 8 + 2 * range, width);
-            const int h = std::min((by - 1) * step + 8 + 2 * range, height);
-            if (2LL * w * h * 8 > kTileLdsBudget) {
-                continue;
-            }
-            const int groups = bx * by;
-            const int passes = (groups + kTileGroupsPerPass - 1) / kTileGroupsPerPass;
-            // idle lanes per group plus a small charge for the scratch traffic
-            const double cost = double(passes * kTileGroupsPerPass) / groups +
-                                0.05 * double(w) * h / (double(bx * step) * (by * step));
-            if (cost < best - 1e-12) {
-                best = cost;
-                *tbx = bx;
-                *tby = by;
-                *rw = w;
-                *rh = h;
-                found = true;
+    const int nx = (width - 8 + step + step - 1) / step, ny = (height - 8 + step + step - 1) / step;
+    auto search = [&](bool whole_launch) {
+        double best = 1e30;
+        bool found = false;
+        for (int by = 1; by <= 32; ++by) {
+            for (int bx = 1; bx <= 32; ++bx) {
+                const int w = std::min((bx - 1) * step + 8 + 2 * range, width);
+                const int h = std::min((by - 1) * step + 8 + 2 * range, height);
+                if (2LL * w * h * 8 > kTileLdsBudget) {
+                    continue;
+                }
+                const int groups = bx * by;
+                const int passes = (groups + kTileGroupsPerPass - 1) / kTileGroupsPerPass;
+                // per group: idle lanes plus a small charge for the scratch
+                // traffic; per launch: rounds of tiles times a tile's cost
+                const double tile = double(passes * kTileGroupsPerPass) + 0.05 * double(w) * h / (double(step) * step);
+                const long long tiles = (long long)((nx + bx - 1) / bx) * ((ny + by - 1) / by);
+                const double cost = whole_launch ? double((tiles + kComputeUnits - 1) / kComputeUnits) * tile
+                                                 : tile / groups;
+                if (cost < best - 1e-12) {
+                    best = cost;
+                    *tbx = bx;
+                    *tby = by;
+                    *rw = w;
+                    *rh = h;
+                    found = true;
+                }
             }
         }
+        return found;
+    };
+    if (!search(false)) {
+        return false;
     }
-    return found;
+    if ((long long)((nx + *tbx - 1) / *tbx) * ((ny + *tby - 1) / *tby) < kComputeUnits) {
+        search(true);
+    }
+    return true;
 }
 
 }  // namespace
