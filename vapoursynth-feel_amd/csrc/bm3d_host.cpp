@@ -192,6 +192,7 @@ struct Instance {
     // the last aggregation that read the result slot it overwrites.
     static constexpr int kLanes = 3, kTemporalLanes = 2;
     static_assert(kTemporalLanes <= kLanes - 1, "radius > 0 puts every lane on a side stream");
+    static_assert(kLanes == 3, "the destructor synchronises stream_l[0] and stream_l[1]");
     uint32_t *groups_l[kLanes - 1] = {};
     float *scratch_l[kLanes - 1] = {};
     uint16_t *psets_l[kLanes - 1] = {};
@@ -482,7 +483,7 @@ struct Instance {
             return;  // creation failed before any device object existed
         }
         (void)hipSetDevice(device);
-        for (hipStream_t q : {stream, copy_in, copy_out}) {
+        for (hipStream_t q : {stream, copy_in, copy_out, stream_l[0], stream_l[1]}) {
             if (q) {
                 (void)hipStreamSynchronize(q);
             }
