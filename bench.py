@@ -5,7 +5,8 @@ A step is one pass of the filter over the whole synthetic clip resident in
 HBM: for the default workload, 32 frames of 1920x1080 GrayS float32 with
 sigma 3, block_step 4, bm_range 16, radius 0 (plain FP32 form).  `value` is
 frames per second over the timed steps (each step's launch sequence replayed
-from the library's HIP graph).  Per-kernel durations come from HIP events
+from the library's HIP graph at radius 0; launched directly at radius > 0,
+the library's default there).  Per-kernel durations come from HIP events
 the library records around every launch on the launch stream, over further
 steps launched directly; the dominant kernel's algorithmic work divided by
 its average duration gives the roofline fraction.  The CPU baseline is the
@@ -148,7 +149,7 @@ def copy_calibration(mb):
 
 def timed(inst, run, steps, warmup):
     """Warm up, time `steps` calls of run() (launch sequences replayed from
-    HIP graphs where the entry point has them), then run max(2, steps // 2)
+    HIP graphs where the entry point replays them by default), then run max(2, steps // 2)
     more calls with the library's per-kernel HIP events on (direct launches)
     for the kernel durations; returns (seconds, {kernel: (avg ms, launches)})."""
     import torch

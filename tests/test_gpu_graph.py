@@ -50,6 +50,7 @@ def test_graph_replays_match_direct_launches(name):
     want = direct.process(x).cpu().numpy()
 
     inst = Instance(nf, h, w, **kw)
+    inst.set_graphs(True)  # (the default at radius 0 only)
     out = torch.empty_like(x)
     for rep in range(3):
         got = inst.process(x, out=out)

@@ -205,7 +205,8 @@ def BM3Dv2(clip: Clip, ref: Clip | None = None, sigma=None, block_step=None, bm_
 
 def set_graphs(node: Clip, requests: bool, device: bool = True) -> None:
     """Graph launches of a BM3Dv2 output node: per-request graphs of its frame
-    requests (off by default) and graph replay of device batches (on)."""
+    requests (off by default) and graph replay of device batches (on by
+    default at radius 0, off at radius > 0)."""
     if not isinstance(node, _Bm3dClip):
         raise TypeError("not a BM3Dv2 node")
     flags = (1 if device else 0) | (2 if requests else 0)
