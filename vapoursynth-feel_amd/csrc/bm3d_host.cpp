@@ -190,21 +190,24 @@ struct Instance {
     // lanes, lane i on stream_l[i], the aggregations on the launch stream,
     // each after the latest estimate of every lane; an estimate waits for
     // the last aggregation that read the result slot it overwrites.
-    static constexpr int kLanes = 3, kTemporalLanes = 2;
-    static_assert(kTemporalLanes <= kLanes - 1, "radius > 0 puts every lane on a side stream");
-    static_assert(kLanes == 3, "the destructor synchronises stream_l[0] and stream_l[1]");
-    uint32_t *groups_l[kLanes - 1] = {};
-    float *scratch_l[kLanes - 1] = {};
-    uint16_t *psets_l[kLanes - 1] = {};
-    uint2 *spatial_l[kLanes - 1] = {};
-    float *origins_l[kLanes - 1] = {};
-    float *rec_l[kLanes - 1] = {};
-    uint32_t *cell_count_l[kLanes - 1] = {};
-    uint2 *cell_list_l[kLanes - 1] = {};
-    float *dst_l[kLanes - 1] = {};  // radius 0: the backward pass's g / D plane
-    hipStream_t stream_l[kLanes - 1] = {};
-    hipEvent_t lane_fork = nullptr, lane_join[kLanes - 1] = {};
-    hipEvent_t lane_done[kLanes] = {};       // radius > 0: a lane's latest estimate
+    // (three lanes at radius > 0: 1047 against 1037 frames/s at 1080p
+    // radius 2 but 365 against 367 on 4K radius 1, for another set of
+    // binning records)
+    static constexpr int kLanes = 3, kTemporalLanes = 2, kMaxLanes = 3;
+    static_assert(kLanes <= kMaxLanes && kTemporalLanes <= kMaxLanes - 1,
+                  "radius > 0 puts every lane on a side stream");
+    uint32_t *groups_l[kMaxLanes - 1] = {};
+    float *scratch_l[kMaxLanes - 1] = {};
+    uint16_t *psets_l[kMaxLanes - 1] = {};
+    uint2 *spatial_l[kMaxLanes - 1] = {};
+    float *origins_l[kMaxLanes - 1] = {};
+    float *rec_l[kMaxLanes - 1] = {};
+    uint32_t *cell_count_l[kMaxLanes - 1] = {};
+    uint2 *cell_list_l[kMaxLanes - 1] = {};
+    float *dst_l[kMaxLanes - 1] = {};  // radius 0: the backward pass's g / D plane
+    hipStream_t stream_l[kMaxLanes - 1] = {};
+    hipEvent_t lane_fork = nullptr, lane_join[kMaxLanes - 1] = {};
+    hipEvent_t lane_done[kMaxLanes] = {};    // radius > 0: a lane's latest estimate
     std::vector<hipEvent_t> slot_read;       // radius > 0: last aggregation reading a result slot
     int lanes = 1;   // lanes of process_device (1: everything on the launch stream)
     int lane = 0;    // the buffer set of the estimate being enqueued
@@ -483,7 +486,12 @@ struct Instance {
             return;  // creation failed before any device object existed
         }
         (void)hipSetDevice(device);
-        for (hipStream_t q : {stream, copy_in, copy_out, stream_l[0], stream_l[1]}) {
+        for (hipStream_t q : {stream, copy_in, copy_out}) {
+            if (q) {
+                (void)hipStreamSynchronize(q);
+            }
+        }
+        for (hipStream_t q : stream_l) {
             if (q) {
                 (void)hipStreamSynchronize(q);
             }
@@ -501,7 +509,7 @@ struct Instance {
         (void)hipFree(cell_count);
         (void)hipFree(cell_list);
         (void)hipFree(psets);
-        for (int i = 0; i < kLanes - 1; ++i) {
+        for (int i = 0; i < kMaxLanes - 1; ++i) {
             (void)hipFree(groups_l[i]);
             (void)hipFree(scratch_l[i]);
             (void)hipFree(psets_l[i]);
@@ -817,7 +825,7 @@ struct Instance {
         Instance &d;
         hipStream_t q;
         int n = 1;
-        bool forked[kLanes] = {};
+        bool forked[kMaxLanes] = {};
         FrameLanes(Instance &inst, hipStream_t s) : d(inst), q(s)
         {
             n = d.timing || d.radius != 0 ? 1 : std::min(d.lanes, d.res_cap);
@@ -1312,9 +1320,9 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
         }
         // VSFEEL_BM3D_LANES caps the lanes (1: one stream and no extra
         // buffer sets, e.g. for per-kernel profiles of kernels running alone)
-        int max_lanes = Instance::kLanes;
+        int max_lanes = Instance::kMaxLanes;
         if (const char *l = std::getenv("VSFEEL_BM3D_LANES")) {
-            max_lanes = std::min(std::max(std::atoi(l), 1), Instance::kLanes);
+            max_lanes = std::min(std::max(std::atoi(l), 1), int(Instance::kMaxLanes));
         }
         if (!d.short_windows && max_lanes > 1) {
             // the lanes' own buffer sets (lane 0 has the instance's)
@@ -1327,7 +1335,8 @@ int vsfeel_bm3d_create(const vsfeel_bm3d_args *args, vsfeel_bm3d **out, char *er
             if ((e = hipEventCreateWithFlags(&d.lane_fork, hipEventDisableTiming)) != hipSuccess) {
                 return dev_fail(e, "hipEventCreate");
             }
-            for (int i = 0; i < Instance::kLanes - 1; ++i) {
+            // side streams: lanes 1.. at radius 0, every lane at radius > 0
+            for (int i = 0; i < (d.radius == 0 ? d.lanes - 1 : d.lanes); ++i) {
                 if ((e = hipStreamCreateWithFlags(&d.stream_l[i], hipStreamNonBlocking)) != hipSuccess ||
                     (e = hipEventCreateWithFlags(&d.lane_join[i], hipEventDisableTiming)) != hipSuccess) {
                     return dev_fail(e, "hipStreamCreate");
@@ -1747,7 +1756,7 @@ int vsfeel_bm3d_process_device(vsfeel_bm3d *h, const vsfeel_bm3d_device_clip *cl
         std::vector<int> held(d.res_cap, -1);
         std::vector<int> slot_of(d.tw);
         std::vector<char> slot_read(d.res_cap, 0);  // slot_read[slot] recorded in this call
-        bool forked[Instance::kLanes] = {}, fresh[Instance::kLanes] = {};
+        bool forked[Instance::kMaxLanes] = {}, fresh[Instance::kMaxLanes] = {};
         // lane j's index in stream_l (-1: the launch stream)
         auto side = [&](int j) { return nlanes == 1 ? -1 : fused ? j - 1 : j; };
         auto lane_stream = [&](int j) { return side(j) < 0 ? q : d.stream_l[side(j)]; };
