@@ -2100,21 +2100,24 @@ hipError_t launch_match(const MatchArgs &a, int fp_mode, hipStream_t s, bool tem
             return hipGetLastError();
         };
         // compiled run geometries: bm_range 16 at block_step 4 (the
-        // benchmark harness) and bm_range 9 at block_step 8 (the plugin's
-        // defaults), where the host derived exactly that geometry
+        // benchmark harness), bm_range 9 at block_step 8 (the plugin's
+        // defaults) and at block_step 4 (the differentiable workload), where
+        // the host derived exactly that geometry
         auto is = [&](int br, int st) {
             const RowsGeometry g = rows_geometry(br, st);
             return a.bm_range == br && a.step == st && a.run_blocks == g.run_blocks &&
                    a.region_cols == g.region_cols && a.region_rows == g.region_rows && a.dist_cols == g.dist_cols;
         };
-        const int geo = is(16, 4) ? 1 : is(9, 8) ? 2 : 0;
+        const int geo = is(16, 4) ? 1 : is(9, 8) ? 2 : is(9, 4) ? 3 : 0;
         if (fused) {
-            return geo == 1 ? go(bm3d_match_rows<kFpAll, 16, 4>)
+            return geo == 1   ? go(bm3d_match_rows<kFpAll, 16, 4>)
                    : geo == 2 ? go(bm3d_match_rows<kFpAll, 9, 8>)
+                   : geo == 3 ? go(bm3d_match_rows<kFpAll, 9, 4>)
                               : go(bm3d_match_rows<kFpAll, 0, 0>);
         }
-        return geo == 1 ? go(bm3d_match_rows<kFpExplicit, 16, 4>)
+        return geo == 1   ? go(bm3d_match_rows<kFpExplicit, 16, 4>)
                : geo == 2 ? go(bm3d_match_rows<kFpExplicit, 9, 8>)
+               : geo == 3 ? go(bm3d_match_rows<kFpExplicit, 9, 4>)
                           : go(bm3d_match_rows<kFpExplicit, 0, 0>);
     }
     MatchArgs b = a;
