@@ -159,6 +159,10 @@ def test_lane_count_leaves_frames_unchanged(name, monkeypatch):
         for graphs in (False, True):
             inst.set_graphs(graphs)
             got[(lanes, graphs)] = inst.process(x).cpu().numpy()
+            # a sub-range: the first estimates of the call on other lanes
+            got[(lanes, graphs, "sub")] = inst.process(x, first=2, count=nf - 3).cpu().numpy()
     want = got[("1", False)]
+    want_sub = got[("1", False, "sub")]
     for key, frames in got.items():
-        assert np.array_equal(frames, want), f"{name}: lanes={key[0]} graphs={key[1]} differs"
+        ref = want_sub if len(key) == 3 else want
+        assert np.array_equal(frames, ref), f"{name}: lanes={key[0]} graphs={key[1]} {key[2:]} differs"
