@@ -150,7 +150,7 @@ def test_lane_count_leaves_frames_unchanged(name, monkeypatch):
     print(f"lanes case {name} clip={clip.shape}", flush=True)
     x = torch.from_numpy(clip).to("cuda:0")
     got = {}
-    for lanes in ("1", "2", None):
+    for lanes in ("1", "2", None, "0", "9"):  # (0 and 9 are clamped to 1 and 3)
         if lanes is None:
             monkeypatch.delenv("VSFEEL_BM3D_LANES", raising=False)
         else:
