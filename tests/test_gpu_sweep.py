@@ -1,0 +1,74 @@
+"""Seeded random configurations against the oracle.
+
+Beyond the fixed parity cases: 30 small clips drawn from one seed (frame
+sizes 40-200 x 32-160, 3-7 frames moving by up to 2 px per frame, block_step
+1-8, bm_range 1-20, radius 0-2, ps_num 1-4, ps_range 1-8, sigma 1/3/6, with
+and without the extractor; every fifth case on the compiled search
+geometries, block_step 4, bm_range 16, ps_num 2, ps_range 4 or 7), run
+through the device path with its default lanes.  Every frame's groups must
+equal the oracle's and the output must lie within 1e-5 of it.  Arguments
+the library rejects are skipped.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+def _configs():
+    rng = np.random.default_rng(20261203)
+    out = []
+    for case in range(30):
+        w = int(rng.integers(40, 200))
+        h = int(rng.integers(32, 160))
+        nf = int(rng.integers(3, 8))
+        kw = dict(sigma=float(rng.choice([1.0, 3.0, 6.0])), block_step=int(rng.integers(1, 9)),
+                  bm_range=int(rng.integers(1, 21)), radius=int(rng.integers(0, 3)),
+                  ps_num=int(rng.integers(1, 5)), ps_range=int(rng.integers(1, 9)),
+                  extractor_exp=int(rng.choice([0, 2])))
+        if case % 5 == 0:
+            kw.update(block_step=4, bm_range=16, ps_num=2, ps_range=int(rng.choice([4, 7])))
+            w, h = max(w, 120), max(h, 96)
+        if kw["block_step"] < 3:
+            w, h = min(w, 96), min(h, 80)
+        shift = (float(rng.uniform(-2, 2)), float(rng.uniform(-2, 2)))
+        out.append((case, nf, w, h, shift, kw))
+    return out
+
+
+@pytest.mark.parametrize("case,nf,w,h,shift,kw", _configs(), ids=[f"case{c[0]}" for c in _configs()])
+def test_random_configuration_matches_oracle(case, nf, w, h, shift, kw):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from oracle import pyoracle
+    from vsfeel import synth
+    from vsfeel.bm3d import Error
+    from vsfeel.torch_bm3d import Instance
+
+    clip = np.ascontiguousarray(synth.shifted_clip(case + 7, nf, w, h, 3 / 255, shift), np.float32)
+    print(f"sweep case {case} clip={clip.shape} args={kw}", flush=True)
+    try:
+        inst = Instance(nf, h, w, **kw)
+    except Error as e:
+        pytest.skip(f"rejected: {e}")
+    x = torch.from_numpy(clip).to("cuda:0")
+    got = inst.process(x).cpu().numpy()
+    p = pyoracle.make_params(w, h, nthreads=NT, **kw)
+    ref = pyoracle.clip(p, clip)
+    r = kw["radius"]
+    for m in range(nf):
+        g, _ = inst.groups(x, m)
+        window = [clip[min(max(m - r + z, 0), nf - 1)] for z in range(2 * r + 1)]
+        _, og = pyoracle.estimate(p, window)
+        n_bad = int((g != og).any(axis=(2, 3)).sum())
+        assert n_bad == 0, f"case {case}: frame {m}: {n_bad} groups differ from the oracle"
+    err = float(np.abs(got - ref).max())
+    assert err <= 1e-5, f"case {case}: output differs from the oracle by {err}"
