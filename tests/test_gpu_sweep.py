@@ -1,4 +1,5 @@
-"""Seeded random configurations against the oracle.
+"""Seeded random configurations against the oracle, and the precision form
+against the plain form.
 
 Beyond the fixed parity cases: 30 small clips drawn from one seed (frame
 sizes 40-200 x 32-160, 3-7 frames moving by up to 2 px per frame, block_step
@@ -72,3 +73,54 @@ def test_random_configuration_matches_oracle(case, nf, w, h, shift, kw):
         assert n_bad == 0, f"case {case}: frame {m}: {n_bad} groups differ from the oracle"
     err = float(np.abs(got - ref).max())
     assert err <= 1e-5, f"case {case}: output differs from the oracle by {err}"
+
+
+def _precision_configs():
+    rng = np.random.default_rng(77)
+    out = []
+    for case in range(16):
+        w = int(rng.integers(48, 220))
+        h = int(rng.integers(40, 160))
+        nf = int(rng.integers(3, 7))
+        kw = dict(sigma=float(rng.choice([1.0, 3.0, 6.0])), block_step=int(rng.integers(2, 9)),
+                  bm_range=int(rng.integers(2, 21)), radius=int(rng.integers(0, 3)),
+                  ps_num=int(rng.integers(1, 4)), ps_range=int(rng.integers(1, 8)))
+        noise = float(rng.choice([1, 3, 8])) / 255
+        shift = (float(rng.uniform(-2, 2)), float(rng.uniform(-2, 2)))
+        out.append((case, nf, w, h, noise, shift, kw))
+    return out
+
+
+@pytest.mark.parametrize("case,nf,w,h,noise,shift,kw", _precision_configs(),
+                         ids=[f"case{c[0]}" for c in _precision_configs()])
+def test_random_configuration_precision_groups(case, nf, w, h, noise, shift, kw):
+    # the precision form (FP16 / BF16 MFMA distances, 16-entry shortlist
+    # rescored exactly) against the plain form's groups: at least 99.9 %
+    # identical (all of them on these cases when this was written)
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from vsfeel import synth
+    from vsfeel.bm3d import Error
+    from vsfeel.torch_bm3d import Instance
+
+    clip = np.ascontiguousarray(synth.shifted_clip(case + 100, nf, w, h, noise, shift), np.float32)
+    print(f"precision sweep case {case} clip={clip.shape} args={kw}", flush=True)
+    x = torch.from_numpy(clip).to("cuda:0")
+    try:
+        plain = Instance(nf, h, w, **kw)
+    except Error as e:
+        pytest.skip(f"rejected: {e}")
+    for prec in ("fp16", "bf16"):
+        try:
+            inst = Instance(nf, h, w, precision=prec, **kw)
+        except Error as e:
+            pytest.skip(f"{prec} rejected: {e}")
+        same = total = 0
+        for m in range(nf):
+            gp, _ = plain.groups(x, m)
+            gm, _ = inst.groups(x, m)
+            eq = (gp == gm).all(axis=(2, 3))
+            same += int(eq.sum())
+            total += eq.size
+        assert same >= 0.999 * total, f"case {case} {prec}: {same}/{total} groups identical to the plain form's"
